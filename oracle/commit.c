@@ -1,0 +1,55 @@
+/*
+ * Oracle: column-wise coset low-degree extension + K12 Merkle commit, and the
+ * seeded input generator.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * The reference has no commit function (pcs exposes only merkle_tree,
+ * pcs/src/k12/mod.rs:41-60).  This composes reference primitives:
+ * intt (ntt/src/ntt/mod.rs:104-120), coefficient scaling by 2717^i
+ * (GENERATOR, ntt/src/field/algo/mod.rs:12), ntt (mod.rs:98-102), the raw
+ * little-endian byte view of a Field slice (bench/src/main.rs:179-183) and
+ * merkle_tree.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
+              uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
+    size_t ext = rows * (size_t)blowup;
+    if (rows == 0 || cols == 0 || blowup == 0 || !or_is_divisor(rows) || !or_is_divisor(ext))
+        return -1;
+    uint64_t *col = (uint64_t *)malloc(ext * sizeof(uint64_t));
+    for (size_t c = 0; c < cols; c++) {
+        for (size_t r = 0; r < rows; r++) col[r] = m[r * cols + c];
+        or_intt(col, rows);
+        memset(col + rows, 0, (ext - rows) * sizeof(uint64_t));
+        uint64_t g = 1;
+        for (size_t r = 0; r < rows; r++) {
+            col[r] = or_mul(col[r], g);
+            g = or_mul(g, OR_GENERATOR);
+        }
+        or_ntt(col, ext);
+        for (size_t r = 0; r < ext; r++) lde_out[r * cols + c] = col[r];
+    }
+    free(col);
+    /* Field words are little-endian u64, so the byte view is the memory. */
+    return or_merkle_tree((const uint8_t *)lde_out, ext * cols * 8, layers_out, cap, out_len);
+}
+
+static uint64_t splitmix64(uint64_t *state) {
+    uint64_t z = (*state += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+void or_fill_random(uint64_t *v, size_t n, uint64_t seed) {
+    for (size_t i = 0; i < n; i++) {
+        uint64_t s = seed + (uint64_t)i * 0x9e3779b97f4a7c15ULL;
+        uint64_t x;
+        do x = splitmix64(&s); while (x >= OR_P);
+        v[i] = x;
+    }
+}

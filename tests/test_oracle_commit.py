@@ -1,0 +1,32 @@
+"""Pins the oracle's commit (column coset LDE + Merkle) at tiny sizes by
+direct polynomial evaluation with Python big integers."""
+import numpy as np
+
+P = 0xFFFFFFFF00000001
+
+
+def test_commit_small_by_evaluation(oracle):
+    rows, cols, blowup = 8, 3, 2
+    m = oracle.fill_random(rows * cols, 77).reshape(rows, cols)
+    lde, layers = oracle.commit(m, blowup)
+    ext = rows * blowup
+    w_rows = pow(2717, (P - 1) // rows, P)
+    w_ext = pow(2717, (P - 1) // ext, P)
+    inv_rows = pow(rows, P - 2, P)
+    for c in range(cols):
+        vals = [int(v) for v in m[:, c]]
+        # coefficients of the interpolant through (w_rows^j, vals[j])
+        coef = [sum(vals[j] * pow(w_rows, (P - 1 - 0) * 0 + (rows - i) * j % rows, P)
+                    for j in range(rows)) * inv_rows % P for i in range(rows)]
+        for j in range(rows):  # the interpolant really interpolates
+            assert sum(coef[i] * pow(w_rows, i * j, P) for i in range(rows)) % P == vals[j]
+        for k in range(ext):
+            x = 2717 * pow(w_ext, k, P) % P
+            assert int(lde[k, c]) == sum(coef[i] * pow(x, i, P) for i in range(rows)) % P
+    assert layers == oracle.merkle_tree(lde.astype("<u8").tobytes())
+
+
+def test_commit_rejects(oracle):
+    import pytest
+    with pytest.raises(ValueError):
+        oracle.commit(np.zeros((7, 2), dtype=np.uint64), 2)

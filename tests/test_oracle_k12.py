@@ -1,0 +1,86 @@
+"""Pins the oracle's Keccak/K12/Merkle against an independent library (hashlib
+SHA3 for the permutation) and the reference's known-answer tests."""
+import hashlib
+import random
+
+import pytest
+
+
+def test_permutation_vs_hashlib(oracle):
+    rng = random.Random(1)
+    for n in [0, 1, 3, 135, 136, 137, 200, 1000] + [rng.randrange(600) for _ in range(20)]:
+        msg = bytes(rng.randrange(256) for _ in range(n))
+        assert oracle.sha3_256(msg) == hashlib.sha3_256(msg).digest()
+
+
+def hx(s):
+    return bytes.fromhex(s.replace(" ", ""))
+
+
+def test_xkcp_kats(oracle):
+    # pcs/src/k12/generic.rs:113-160 (XKCP KangarooTwelve.txt)
+    assert oracle.k12(b"", 32) == hx(
+        "1a c2 d4 50 fc 3b 42 05 d1 9d a7 bf ca 1b 37 51 3c 08 03 57 7a c7 16 7f 06 fe 2c e1 f0 ef 39 e5")
+    assert oracle.k12(b"", 64) == hx(
+        "1a c2 d4 50 fc 3b 42 05 d1 9d a7 bf ca 1b 37 51 3c 08 03 57 7a c7 16 7f 06 fe 2c e1 "
+        "f0 ef 39 e5 42 69 c0 56 b8 c8 2e 48 27 60 38 b6 d2 92 96 6c c0 7a 3d 46 45 27 2e 31 "
+        "ff 38 50 81 39 eb 0a 71")
+    ptn = lambda n: bytes(i % 0xFA for i in range(n))
+    assert oracle.k12(ptn(1), 32) == hx(
+        "2b da 92 45 0e 8b 14 7f 8a 7c b6 29 e7 84 a0 58 ef ca 7c f7 d8 21 8e 02 d3 45 df aa 65 24 4a 1f")
+    assert oracle.k12(ptn(17), 32) == hx(
+        "6b f7 5f a2 23 91 98 db 47 72 e3 64 78 f8 e1 9b 0f 37 12 05 f6 a9 a9 3a 27 3f 51 df 37 12 28 88")
+
+
+def test_k12_limits(oracle):
+    oracle.k12(bytes(166), 32)
+    with pytest.raises(ValueError):
+        oracle.k12(bytes(167), 32)
+
+
+def test_pow_kats(oracle):
+    ch = bytes([0x55] * 32)
+    # pow.rs:67-85
+    assert all(oracle.pow_verify(ch, 0, i) for i in range(4))
+    assert [oracle.pow_verify(ch, 1 << 63, i) for i in range(4)] == [False, False, False, True]
+    # pow.rs:87-99
+    assert oracle.pow_solve(ch, 1 << 63) == 3
+    assert oracle.pow_solve(ch, (2**64 - 1) - 2**48) == 18455
+
+    # k12/aarch64.rs:234-240: k12_pow_2([0x55;32], u64::MAX, 0)
+    def first_word(nonce):
+        h = oracle.k12(ch + nonce.to_bytes(8, "little"), 8)
+        return int.from_bytes(h, "little")
+    assert first_word(2**64 - 1) == 14608096357653754266
+    assert first_word(0) == 2237294155800119683
+
+
+def test_layers(oracle):
+    assert oracle.next_layer_size(0) == 0
+    assert oracle.next_layer_size(1) == 32
+    assert oracle.next_layer_size(160) == 32
+    assert oracle.next_layer_size(161) == 64
+    data = bytes(i % 251 for i in range(1600 + 37))
+    out = oracle.process_layer(data)
+    assert len(out) == 11 * 32
+    for i in range(11):
+        assert out[32 * i:32 * i + 32] == oracle.k12(data[160 * i:160 * i + 160], 32)
+
+
+def test_merkle_tree(oracle):
+    # mod.rs:75-84: every layer is a multiple of 32 bytes; stop at <= 32.
+    data = bytes(i % 0xFA for i in range(1 << 20))
+    layers = oracle.merkle_tree(data)
+    assert all(len(l) % 32 == 0 for l in layers)
+    assert len(layers[-1]) == 32
+    sizes, n = [], len(data)
+    while n > 32:
+        n = oracle.next_layer_size(n)
+        sizes.append(n)
+    assert [len(l) for l in layers] == sizes
+    prev = data
+    for l in layers:
+        assert l == oracle.process_layer(prev)
+        prev = l
+    assert oracle.merkle_tree(b"") == [] and oracle.merkle_tree(bytes(32)) == []
+    assert len(oracle.merkle_tree(bytes(33))) == 1
