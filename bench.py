@@ -1,0 +1,145 @@
+"""Benchmark: Goldilocks forward NTT of 2^24 words on one MI355X.
+
+A step is one forward NTT over 2^24 HBM-resident words (synthetic, seeded
+uniform in [0, p)).  Prints one JSON line with the whole-job throughput
+(Gelem/s), the roofline of the dominant kernel and a CPU baseline (the
+oracle's port of the reference algorithm on this host's cores, bounded
+sample).  Extra workloads: --workload batch (mixed-size batched NTTs,
+batched vs one-launch-sequence-per-transform, with p50/p99 latency),
+--workload commit (coset LDE + K12 Merkle commit).
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bound on the CPU baseline sample (0 disables it)")
+    return ap.parse_args()
+
+
+def traffic_for(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-kernel calibrated), if recorded for this kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline_ntt(n, seconds):
+    """The oracle (C port of the reference plans) on this host, 1 thread,
+    repeated whole 2^k transforms until `seconds` of CPU time."""
+    if seconds <= 0:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    x = oracle.fill_random(n, 1)
+    times = []
+    t_end = time.perf_counter() + seconds
+    while True:
+        t0 = time.perf_counter()
+        oracle.ntt(x)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() >= t_end or len(times) >= 50:
+            break
+    med = statistics.median(times)
+    return {"value": n / med / 1e9, "unit": "Gelem/s", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} forward NTTs of 2^{n.bit_length() - 1} words (median)"}
+
+
+def bench_ntt(args, torch, gl):
+    n = 1 << args.log_n
+    x = torch.empty(n, dtype=torch.int64, device="cuda")
+    gl.fill_random(x, 1)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        gl.ntt(x)
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record(stream)
+    for i in range(args.steps):
+        gl.ntt(x)
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    total_ms = evs[0].elapsed_time(evs[-1])
+    ms = total_ms / args.steps
+    per_sorted = sorted(per)
+    p50 = per_sorted[len(per) // 2]
+    p99 = per_sorted[min(len(per) - 1, int(0.99 * len(per)))]
+    plan = gl.plan_info(n)
+    passes = len(plan)
+    kname = plan[0]["kernel"] if plan else "none"
+    # Every pass reads and writes each word once: 16 n algorithmic bytes per launch.
+    per_launch_ms = ms / max(passes, 1)
+    achieved = 16.0 * n / (per_launch_ms * 1e-3) / 1e9
+    gelem = n / (ms * 1e-3) / 1e9
+    return {
+        "metric": "Gelem/s, 2^24-point forward Goldilocks NTT (HBM-resident)",
+        "value": round(gelem, 3),
+        "unit": "Gelem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(gelem / 0.14, 1),
+        "vs_baseline_ref": "M1 Max CPU ntt-par at n~2^24, ~1.4e8 elem/s (bench/analysis.ipynb plot)",
+        "dtype": "u64",
+        "data": "synthetic: seeded splitmix64 words uniform in [0,p)",
+        "config": {"workload": f"forward NTT, n=2^{args.log_n}, in place, words resident in HBM",
+                   "n": n, "passes": [p["R"] for p in plan]},
+        "latency_ms": {"p50": round(p50, 5), "p99": round(p99, 5)},
+        "transform_hbm_frac": round(16.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_for(kname)},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import goldilocks_amd as gl
+    torch.cuda.set_device(0)
+    rc = gl.lib().gl_init(0)
+    if rc:
+        raise SystemExit(gl.lib().gl_last_error().decode())
+    if args.workload == "ntt":
+        out = bench_ntt(args, torch, gl)
+        out["cpu_baseline"] = cpu_baseline_ntt(1 << args.log_n, args.cpu_seconds)
+    elif args.workload == "batch":
+        from goldilocks_amd.benchmarks import bench_batch
+        out = bench_batch(args, torch, gl)
+    else:
+        from goldilocks_amd.benchmarks import bench_commit
+        out = bench_commit(args, torch, gl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

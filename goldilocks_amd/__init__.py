@@ -1,0 +1,303 @@
+"""Goldilocks NTT and K12 Merkle commit on MI355X (gfx950).
+
+Python mirror of the reference's entry points over the C ABI in
+include/goldilocks_gpu.h (libgoldilocks_amd.so, loaded with ctypes):
+
+  ntt crate (ntt/src/ntt/mod.rs)    ntt(values), intt(values), strategy(n),
+                                    clear_cache(), is_divisor(n)
+  pcs crate (pcs/src/k12/mod.rs)    merkle_tree(data), process_layer(input),
+                                    k12(data, H), next_layer_size(n)
+  composed                          commit(matrix, blowup), ntt_batch_mixed(list)
+
+`values` is a numpy uint64 array (host: copied to HBM and back) or a torch
+tensor on the GPU (int64/uint64 storage, transformed in place on the current
+stream).  Field words are the raw u64 words of the reference's `[Field]`.
+Where the reference panics these functions raise ValueError; device errors
+raise RuntimeError.  There is no CPU fallback: if the HIP library is missing
+or no gfx950 device is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "P", "ntt", "intt", "strategy", "Strategy", "clear_cache", "is_divisor",
+    "merkle_tree", "process_layer", "k12", "next_layer_size", "merkle_tree_size",
+    "commit", "ntt_batch_mixed", "fill_random", "lib", "GoldilocksError",
+]
+
+P = 0xFFFFFFFF00000001
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgoldilocks_amd.so")
+
+GL_OK, GL_E_SIZE, GL_E_NOMEM, GL_E_DEVICE, GL_E_ARG, GL_E_UNSUPPORTED, GL_E_BUSY = range(7)
+
+
+class GoldilocksError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; run `python -m goldilocks_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        u64, sz, i32, vp = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
+        psz = ctypes.POINTER(sz)
+        sigs = {
+            "gl_last_error": (ctypes.c_char_p, []),
+            "gl_init": (i32, [i32]),
+            "gl_is_divisor": (i32, [u64]),
+            "gl_ntt": (i32, [vp, sz]),
+            "gl_intt": (i32, [vp, sz]),
+            "gl_ntt_batch": (i32, [vp, sz, sz, i32]),
+            "gl_ntt_batch_mixed": (i32, [vp, vp, sz, i32]),
+            "gl_ntt_dev": (i32, [vp, sz, sz, i32, vp]),
+            "gl_ntt_cols_dev": (i32, [vp, sz, sz, sz, i32, vp]),
+            "gl_ntt_mixed_dev": (i32, [vp, vp, sz, i32, vp]),
+            "gl_ntt_mixed_dev_unbatched": (i32, [vp, vp, sz, i32, vp]),
+            "gl_fill_random_dev": (i32, [vp, sz, u64, vp]),
+            "gl_clear_cache": (None, []),
+            "gl_plan_passes": (i32, [u64, vp, vp, vp, sz, psz]),
+            "gl_k12": (i32, [vp, sz, vp, sz]),
+            "gl_next_layer_size": (sz, [sz]),
+            "gl_merkle_tree_size": (sz, [sz]),
+            "gl_k12_layer": (i32, [vp, sz, vp]),
+            "gl_k12_layer_dev": (i32, [vp, sz, vp, vp]),
+            "gl_merkle_tree": (i32, [vp, sz, vp, sz, psz]),
+            "gl_merkle_tree_dev": (i32, [vp, sz, vp, sz, psz, vp]),
+            "gl_commit": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz]),
+            "gl_commit_dev": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc == GL_OK:
+        return
+    msg = lib().gl_last_error().decode(errors="replace")
+    if rc in (GL_E_SIZE, GL_E_ARG):
+        raise ValueError(msg)
+    raise GoldilocksError(rc, msg)
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _stream_ptr(t):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _host_u64(values):
+    if not isinstance(values, np.ndarray) or values.dtype != np.uint64 or not values.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous numpy uint64 array")
+    return values
+
+
+def _dev_ptr(t):
+    if not t.is_cuda or t.element_size() != 8 or not t.is_contiguous():
+        raise TypeError("expected a contiguous 64-bit torch tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def is_divisor(n):
+    """ntt/src/divisors.rs:21-23."""
+    return bool(lib().gl_is_divisor(n))
+
+
+def _transform(values, n, count, inverse):
+    if _is_torch(values):
+        _check(lib().gl_ntt_dev(_dev_ptr(values), n, count, int(inverse), _stream_ptr(values)))
+    else:
+        v = _host_u64(values)
+        _check(lib().gl_ntt_batch(v.ctypes.data_as(ctypes.c_void_p), n, count, int(inverse)))
+    return values
+
+
+def ntt(values):
+    """Forward NTT in place (ntt/src/ntt/mod.rs:98-102)."""
+    return _transform(values, _len(values), 1, False)
+
+
+def intt(values):
+    """Inverse NTT in place (ntt/src/ntt/mod.rs:104-120)."""
+    return _transform(values, _len(values), 1, True)
+
+
+def _len(values):
+    return int(values.numel()) if _is_torch(values) else int(values.size)
+
+
+class Strategy:
+    """strategy(n) (ntt/src/ntt/mod.rs:48-78): an NTT of length n applied to
+    every consecutive chunk of n words (trait Ntt, mod.rs:25-31)."""
+
+    def __init__(self, n):
+        if not is_divisor(n):
+            raise ValueError(f"{n} is not a supported NTT size (does not divide multiplicative order)")
+        self.n = n
+
+    def len(self):
+        return self.n
+
+    def __len__(self):
+        return self.n
+
+    def ntt(self, values, inverse=False):
+        total = _len(values)
+        if total % self.n:
+            raise ValueError("values.len() must be a multiple of the NTT length")
+        return _transform(values, self.n, total // self.n, inverse)
+
+
+def strategy(n):
+    return Strategy(n)
+
+
+def clear_cache():
+    """ntt/src/ntt/mod.rs:43-46 (drops device plans and twiddle tables too)."""
+    lib().gl_clear_cache()
+
+
+def plan_info(n):
+    """Device schedule for length n: [{"R1", "R2", "R", "T", "kernel"}, ...]."""
+    cap = 8
+    r1, r2, t = (ctypes.c_uint32 * cap)(), (ctypes.c_uint32 * cap)(), (ctypes.c_uint32 * cap)()
+    npass = ctypes.c_size_t(0)
+    _check(lib().gl_plan_passes(n, r1, r2, t, cap, ctypes.byref(npass)))
+    return [{"R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
+             "kernel": f"ntt_pass_kernel<{r1[i]}, {r2[i]}, 256, {t[i]}>"} for i in range(npass.value)]
+
+
+def ntt_cols(matrix, inverse=False):
+    """Transform each column of a row-major (rows x cols) GPU tensor in place."""
+    rows, cols = matrix.shape
+    _check(lib().gl_ntt_cols_dev(_dev_ptr(matrix), rows, cols, 1, int(inverse), _stream_ptr(matrix)))
+    return matrix
+
+
+def ntt_batch_mixed(arrays, inverse=False, batched=True):
+    """Transform every array of a list in place (mixed lengths/radices).
+    Host numpy arrays or GPU tensors (all of one kind)."""
+    k = len(arrays)
+    if k == 0:
+        return arrays
+    ns = (ctypes.c_size_t * k)(*[_len(a) for a in arrays])
+    if _is_torch(arrays[0]):
+        ptrs = (ctypes.c_void_p * k)(*[_dev_ptr(a).value for a in arrays])
+        fn = lib().gl_ntt_mixed_dev if batched else lib().gl_ntt_mixed_dev_unbatched
+        _check(fn(ptrs, ns, k, int(inverse), _stream_ptr(arrays[0])))
+    else:
+        ptrs = (ctypes.c_void_p * k)(*[_host_u64(a).ctypes.data for a in arrays])
+        _check(lib().gl_ntt_batch_mixed(ptrs, ns, k, int(inverse)))
+    return arrays
+
+
+def fill_random(tensor, seed):
+    """Seeded uniform [0, p) words generated on the GPU (same map as the tests)."""
+    _check(lib().gl_fill_random_dev(_dev_ptr(tensor), _len(tensor), seed, _stream_ptr(tensor)))
+    return tensor
+
+
+# ---------------------------------------------------------------- K12 / Merkle
+
+def next_layer_size(n):
+    """pcs/src/k12/mod.rs:33-35."""
+    return lib().gl_next_layer_size(n)
+
+
+def merkle_tree_size(n):
+    return lib().gl_merkle_tree_size(n)
+
+
+def k12(data: bytes, H: int = 32) -> bytes:
+    """k12::<H>(input) (pcs/src/k12/generic.rs:62-84); len(data) <= 166."""
+    data = bytes(data)
+    out = ctypes.create_string_buffer(max(H, 1))
+    _check(lib().gl_k12(data, len(data), out, H))
+    return out.raw[:H]
+
+
+def process_layer(data: bytes) -> bytes:
+    """process_layer (pcs/src/k12/generic.rs:86-94): one 32-byte K12 hash per
+    160-byte block (last block may be short)."""
+    data = bytes(data)
+    out = ctypes.create_string_buffer(max(1, next_layer_size(len(data))))
+    _check(lib().gl_k12_layer(data, len(data), out))
+    return out.raw[: next_layer_size(len(data))]
+
+
+def _split_layers(raw, n):
+    layers, off = [], 0
+    while n > 32:
+        n = next_layer_size(n)
+        layers.append(raw[off:off + n])
+        off += n
+    return layers
+
+
+def merkle_tree(data):
+    """merkle_tree (pcs/src/k12/mod.rs:41-60): list of layers, root last.
+
+    bytes / numpy input: hashed on the GPU, layers returned as bytes.
+    GPU tensor input (any dtype, contiguous): layers returned as one uint8
+    GPU tensor of all layers concatenated (split with next_layer_size)."""
+    if _is_torch(data):
+        import torch
+        nbytes = data.numel() * data.element_size()
+        need = merkle_tree_size(nbytes)
+        out = torch.empty(max(need, 1), dtype=torch.uint8, device=data.device)
+        got = ctypes.c_size_t(0)
+        _check(lib().gl_merkle_tree_dev(ctypes.c_void_p(data.data_ptr()), nbytes,
+                                        ctypes.c_void_p(out.data_ptr()), need, ctypes.byref(got),
+                                        _stream_ptr(data)))
+        return out[:need]
+    if isinstance(data, np.ndarray):
+        data = np.ascontiguousarray(data).tobytes()
+    data = bytes(data)
+    need = merkle_tree_size(len(data))
+    out = ctypes.create_string_buffer(max(need, 1))
+    got = ctypes.c_size_t(0)
+    _check(lib().gl_merkle_tree(data, len(data), out, need, ctypes.byref(got)))
+    return _split_layers(out.raw[:need], len(data))
+
+
+def commit(matrix, blowup=2):
+    """Column coset LDE (intt, zero-extend, x GENERATOR^i, ntt) of a row-major
+    rows x cols matrix, then merkle_tree over the LDE bytes.
+
+    numpy input -> (lde ndarray, [layer bytes]); GPU tensor input ->
+    (lde tensor, layers uint8 tensor)."""
+    rows, cols = matrix.shape
+    ext = rows * blowup
+    need = merkle_tree_size(ext * cols * 8)
+    got = ctypes.c_size_t(0)
+    if _is_torch(matrix):
+        import torch
+        lde = torch.empty((ext, cols), dtype=matrix.dtype, device=matrix.device)
+        layers = torch.empty(max(need, 1), dtype=torch.uint8, device=matrix.device)
+        _check(lib().gl_commit_dev(_dev_ptr(matrix), rows, cols, blowup, _dev_ptr(lde),
+                                   ctypes.c_void_p(layers.data_ptr()), need, ctypes.byref(got),
+                                   _stream_ptr(matrix)))
+        return lde, layers[:need]
+    m = _host_u64(np.ascontiguousarray(matrix, dtype=np.uint64))
+    lde = np.zeros((ext, cols), dtype=np.uint64)
+    out = ctypes.create_string_buffer(max(need, 1))
+    _check(lib().gl_commit(m.ctypes.data_as(ctypes.c_void_p), rows, cols, blowup,
+                           lde.ctypes.data_as(ctypes.c_void_p), out, need, ctypes.byref(got)))
+    return lde, _split_layers(out.raw[:need], ext * cols * 8)

@@ -1,0 +1,269 @@
+// C ABI (include/goldilocks_gpu.h): host-buffer and device-buffer entry
+// points for the NTT, K12 layers, Merkle trees and the commit.
+#include <cstring>
+#include <vector>
+
+#include "gl_field.hpp"
+#include "gl_runtime.hpp"
+
+using namespace gl;
+
+namespace gl {
+const char *last_error_cstr();
+}
+
+namespace {
+
+constexpr size_t HASH = 32, BLOCK = 160;
+
+size_t next_layer(size_t len) { return (len + BLOCK - 1) / BLOCK * HASH; }
+
+size_t tree_size(size_t len) {
+    size_t total = 0;
+    while (len > HASH) {
+        len = next_layer(len);
+        total += len;
+    }
+    return total;
+}
+
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (inverse && n <= 1) return GL_OK;  // mod.rs:105-107
+    const Plan *plan = nullptr;
+    if ((rc = get_plan(n, &plan))) return rc;
+    if (count == 0 || S0 == 0 || plan->passes.empty()) return GL_OK;
+    if (!d) return set_error(GL_E_ARG, "null device pointer");
+    const u64 bstride = n * S0;
+    size_t words = plan_tmp_words(*plan, count, bstride);
+    u64 *tmp = stream_scratch(s, words);
+    if (words && !tmp) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    return run_plan(*plan, d, tmp, S0, count, bstride, inverse, s);
+}
+
+int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
+                    hipStream_t s) {
+    size_t need = tree_size(len);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
+    const uint8_t *layer = d_in;
+    uint8_t *dst = d_out;
+    while (len > HASH) {
+        launch_k12_layer(layer, len, dst, s);
+        GL_HIP(hipGetLastError());
+        layer = dst;
+        len = next_layer(len);
+        dst += len;
+    }
+    return GL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gl_last_error(void) { return last_error_cstr(); }
+
+int gl_init(int device) {
+    if (device >= 0) GL_HIP(hipSetDevice(device));
+    return ensure_init();
+}
+
+int gl_is_divisor(uint64_t n) { return n > 0 && (P - 1) % n == 0; }
+
+// ---------------------------------------------------------------- NTT
+
+int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream) {
+    return ntt_dev_impl(d, n, 1, count, inverse != 0, as_stream(stream));
+}
+
+int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int inverse, void *stream) {
+    return ntt_dev_impl(d, n, cols, count, inverse != 0, as_stream(stream));
+}
+
+int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (inverse && n <= 1) return GL_OK;
+    if (!gl_is_divisor(n)) return get_plan(n, nullptr);
+    if (count == 0) return GL_OK;
+    if (!v) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    size_t bytes = n * count * 8;
+    DevBuf d(bytes);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    GL_HIP(hipMemcpyAsync(d.p, v, bytes, hipMemcpyHostToDevice, s));
+    if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s))) return rc;
+    GL_HIP(hipMemcpyAsync(v, d.p, bytes, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+int gl_ntt(uint64_t *v, size_t n) { return gl_ntt_batch(v, n, 1, 0); }
+int gl_intt(uint64_t *v, size_t n) { return gl_ntt_batch(v, n, 1, 1); }
+
+int gl_fill_random_dev(uint64_t *d, size_t n, uint64_t seed, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n && !d) return set_error(GL_E_ARG, "null device pointer");
+    launch_fill_random(d, n, seed, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+void gl_clear_cache(void) { clear_caches(); }
+
+int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses) {
+    const Plan *plan = nullptr;
+    int rc = get_plan(n, &plan);
+    if (rc) return rc;
+    if (npasses) *npasses = plan->passes.size();
+    for (size_t i = 0; i < plan->passes.size() && i < cap; i++) {
+        if (r1) r1[i] = (uint32_t)plan->passes[i].v->R1;
+        if (r2) r2[i] = (uint32_t)plan->passes[i].v->R2;
+        if (t) t[i] = (uint32_t)plan->passes[i].v->T;
+    }
+    return GL_OK;
+}
+
+// ---------------------------------------------------------------- K12 / Merkle
+
+size_t gl_next_layer_size(size_t len) { return next_layer(len); }
+size_t gl_merkle_tree_size(size_t len) { return tree_size(len); }
+
+int gl_k12(const uint8_t *in, size_t len, uint8_t *out, size_t outlen) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (len > 166 || outlen > 168) return set_error(GL_E_ARG, "k12: input > 166 bytes or output > 168 bytes");
+    if ((len && !in) || (outlen && !out)) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf d(256 + 256);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    uint8_t *din = d.as<uint8_t>(), *dout = din + 256;
+    if (len) GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
+    launch_k12_single(din, len, dout, outlen, s);
+    GL_HIP(hipGetLastError());
+    if (outlen) GL_HIP(hipMemcpyAsync(out, dout, outlen, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+int gl_k12_layer_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (len && (!d_in || !d_out)) return set_error(GL_E_ARG, "null device pointer");
+    launch_k12_layer(d_in, len, d_out, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (len == 0) return GL_OK;
+    if (!in || !out) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    size_t olen = next_layer(len);
+    DevBuf d(len + olen + 256);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    uint8_t *din = d.as<uint8_t>(), *dout = din + ((len + 255) / 256 * 256);
+    GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
+    launch_k12_layer(din, len, dout, s);
+    GL_HIP(hipGetLastError());
+    GL_HIP(hipMemcpyAsync(out, dout, olen, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+int gl_merkle_tree_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
+                       void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (len > HASH && (!d_in || !d_out)) return set_error(GL_E_ARG, "null device pointer");
+    return merkle_dev_impl(d_in, len, d_out, cap, out_len, as_stream(stream));
+}
+
+int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size_t *out_len) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    size_t need = tree_size(len);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
+    if (need == 0) return GL_OK;
+    if (!in || !out) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    size_t in_pad = (len + 255) / 256 * 256;
+    DevBuf d(in_pad + need);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
+    GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
+    if ((rc = merkle_dev_impl(din, len, dout, need, nullptr, s))) return rc;
+    GL_HIP(hipMemcpyAsync(out, dout, need, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+// ---------------------------------------------------------------- commit
+
+int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde,
+                  uint8_t *d_layers, size_t cap, size_t *out_len, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const u64 ext = (u64)rows * blowup;
+    if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
+    if (!gl_is_divisor(rows) || !gl_is_divisor(ext))
+        return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
+    size_t need = tree_size(ext * cols * 8);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
+    if (!d_m || !d_lde || !d_layers) return set_error(GL_E_ARG, "null device pointer");
+    const Plan *prow = nullptr, *pext = nullptr;
+    if ((rc = get_plan(rows, &prow))) return rc;
+    if ((rc = get_plan(ext, &pext))) return rc;
+    // coefficients: column intt of a copy of the input
+    DevBuf coef(rows * cols * 8);
+    DevBuf tmp(ext * cols * 8);
+    if (!coef.p || !tmp.p) return set_error(GL_E_NOMEM, "commit scratch allocation failed");
+    GL_HIP(hipMemcpyAsync(coef.p, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
+    if ((rc = run_plan(*prow, coef.as<u64>(), tmp.as<u64>(), cols, 1, rows * cols, true, s))) return rc;
+    PowTable g = power_table(2717, rows);
+    if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    launch_coset_extend(coef.as<u64>(), d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
+    GL_HIP(hipGetLastError());
+    if ((rc = run_plan(*pext, d_lde, tmp.as<u64>(), cols, 1, ext * cols, false, s))) return rc;
+    if ((rc = merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, cap, nullptr, s)))
+        return rc;
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
+              uint8_t *layers_out, size_t cap, size_t *out_len) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    const u64 ext = (u64)rows * blowup;
+    if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
+    if (!gl_is_divisor(rows) || !gl_is_divisor(ext))
+        return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
+    size_t need = tree_size(ext * cols * 8);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
+    if (!m || !lde_out || !layers_out) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf dm(rows * cols * 8), dl(ext * cols * 8), dt(need + 256);
+    if (!dm.p || !dl.p || !dt.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    GL_HIP(hipMemcpyAsync(dm.p, m, rows * cols * 8, hipMemcpyHostToDevice, s));
+    if ((rc = gl_commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need,
+                            nullptr, s)))
+        return rc;
+    GL_HIP(hipMemcpyAsync(lde_out, dl.p, ext * cols * 8, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipMemcpyAsync(layers_out, dt.p, need, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+}  // extern "C"

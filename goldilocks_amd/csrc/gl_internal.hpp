@@ -1,0 +1,69 @@
+// Internal declarations shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gl {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// One Stockham decimation-in-frequency pass of radix R = R1 * R2.
+//
+// The array holds `count` independent items of `bstride` words.  Inside an
+// item the pass sees S interleaved sub-problems of length M = R * Mp; word
+// (s, n) of sub-problem s sits at s + S*n.  Instance i = s + S*m (m < Mp)
+// gathers x_r = in[i + S*Mp*r] (r < R), computes y = DFT_R(x), multiplies
+// y_k by w_M^(m*k) and scatters to out[s + S*k + S*R*m].  After the last
+// pass (Mp = 1) the item is in natural order (a standard Stockham autosort
+// ordering, so no separate transpose pass is needed).
+struct PassArgs {
+    const u64 *in;
+    u64 *out;
+    u64 bstride;       // words per batch item
+    u64 S;             // interleaved sub-problems
+    u64 SMp;           // S * Mp: input stride between r and r+1
+    u64 n_rev;         // transform length when reversing the input (intt)
+    u32 ninst;         // S * Mp instances per item
+    u32 total_inst;    // count * ninst
+    u32 Mp;
+    u32 lo_bits;       // output twiddle w_M^e = lo[e & mask] * hi[e >> lo_bits]
+    const u64 *tw_inner;  // w_R^e, e < R
+    const u64 *tw_lo;
+    const u64 *tw_hi;
+    u64 scale;         // multiplier applied to every output when FLAG_SCALE
+    u32 flags;
+};
+
+enum : u32 {
+    FLAG_TWIDDLE = 1u,  // apply w_M^(m*k) to the outputs (Mp > 1)
+    FLAG_REVERSE = 2u,  // read x[(n - j) mod n] (first pass of intt)
+    FLAG_SCALE = 4u,    // multiply outputs by `scale` (1/n for intt)
+};
+
+struct PassVariant {
+    int R1, R2, T, NT;
+    size_t lds_bytes;
+    void (*launch)(const PassArgs &, unsigned grid, hipStream_t);
+    // descs/starts in device memory; descriptor j owns blocks [starts[j], starts[j+1])
+    void (*launch_grouped)(const PassArgs *descs, const u32 *starts, u32 nd, unsigned grid, hipStream_t);
+};
+
+// All compiled pass variants (gl_ntt_kernels.hip).
+const PassVariant *pass_variants(int *count);
+
+// Elementwise kernels (gl_ntt_kernels.hip).
+void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
+void launch_scale_reverse(u64 *v, u64 n, u64 count, u64 scale, hipStream_t s);
+void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
+                         const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
+
+// K12 kernels (gl_k12.hip).
+void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, hipStream_t s);
+void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hipStream_t s);
+void launch_pow_search(const uint8_t *challenge32, u64 threshold, u64 start, u64 count,
+                       unsigned long long *best, hipStream_t s);
+
+}  // namespace gl

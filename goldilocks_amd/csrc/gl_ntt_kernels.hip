@@ -1,0 +1,268 @@
+// NTT pass kernels for gfx950 (CDNA4, wave64).
+//
+// A pass computes one Stockham DIF step of radix R = R1*R2 for T consecutive
+// instances per workgroup (see PassArgs).  The radix-R DFT of an instance is
+// split as R = R1 x R2 (ntt/src/ntt/cooley_tukey.rs:80-148 restated for one
+// level): sub-step a runs DFT_R1 in registers, multiplies by w_R^(ka*rb) and
+// writes to LDS; sub-step b reads the transposed view, runs DFT_R2 in
+// registers, applies the pass's output twiddle w_M^(m*k) and stores.
+//
+// Memory: lanes map to consecutive instances first, so every global load and
+// store instruction touches runs of T (or more) consecutive words; each word
+// is read once and written once per pass.  LDS rows are padded (odd stride
+// per instance) and XOR-swizzled (power-of-two R2) so the transposed reads
+// of sub-step b do not serialize on banks.
+#include "gl_dft.hpp"
+#include "gl_internal.hpp"
+
+namespace gl {
+
+template <int R2>
+__device__ __forceinline__ int lds_col(int rb, int ka) {
+    if constexpr (is_pow2(R2)) return rb ^ (ka & (R2 - 1));
+    else return rb;
+}
+
+template <int R1, int R2, int T>
+constexpr int lds_words() { return R2 > 1 ? T * (R1 * R2 + 1) : 1; }
+
+// One workgroup's share of a pass: instances [blk*T, blk*T + T).
+template <int R1, int R2, int NT, int T>
+__device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) {
+    constexpr int R = R1 * R2;
+    constexpr int TS = R + 1;
+    const int tid = threadIdx.x;
+    const u32 I0 = blk * (u32)T;
+
+    constexpr int TA = T * R2;
+    constexpr int QA = (TA + NT - 1) / NT;
+#pragma unroll 1
+    for (int q = 0; q < QA; q++) {
+        const int ta = tid + q * NT;
+        if (ta >= TA) break;
+        const int t = ta % T, rb = ta / T;
+        const u32 I = I0 + (u32)t;
+        const bool valid = I < a.total_inst;
+        u64 x[R1];
+        u32 b = 0, s = 0, m = 0;
+        if (valid) {
+            b = I / a.ninst;
+            const u32 i = I - b * a.ninst;
+            s = (u32)(i % (u32)a.S);
+            m = (u32)(i / (u32)a.S);
+            const u64 *src = a.in + (u64)b * a.bstride;
+            if (a.flags & FLAG_REVERSE) {
+#pragma unroll
+                for (int ra = 0; ra < R1; ra++) {
+                    u64 pos = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+                    u64 rpos = pos ? a.n_rev - pos : 0;
+                    x[ra] = src[s + a.S * rpos];
+                }
+            } else {
+                const u64 *p = src + (u64)(s + a.S * (u64)m) + a.SMp * (u64)rb;
+#pragma unroll
+                for (int ra = 0; ra < R1; ra++) x[ra] = p[a.SMp * (u64)(ra * R2)];
+            }
+        } else {
+#pragma unroll
+            for (int ra = 0; ra < R1; ra++) x[ra] = 0;
+        }
+        dft<R1>(x);
+        if constexpr (R2 > 1) {
+#pragma unroll
+            for (int ka = 1; ka < R1; ka++) x[ka] = mul(x[ka], a.tw_inner[(ka * rb) % R]);
+            u64 *row = lds + t * TS;
+#pragma unroll
+            for (int ka = 0; ka < R1; ka++) row[ka * R2 + lds_col<R2>(rb, ka)] = x[ka];
+        } else if (valid) {
+            // single sub-step pass: k = ka
+            u64 *dst = a.out + (u64)b * a.bstride + s + a.S * (u64)R * (u64)m;
+#pragma unroll
+            for (int k = 0; k < R1; k++) {
+                u64 v = x[k];
+                if (a.flags & FLAG_TWIDDLE) {
+                    u64 e = (u64)m * (u64)k;
+                    v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                }
+                if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                dst[a.S * (u64)k] = v;
+            }
+        }
+    }
+    if constexpr (R2 > 1) {
+        __syncthreads();
+        constexpr int TB = T * R1;
+        constexpr int QB = (TB + NT - 1) / NT;
+#pragma unroll 1
+        for (int q = 0; q < QB; q++) {
+            const int tb = tid + q * NT;
+            if (tb >= TB) break;
+            const int t = tb % T, ka = tb / T;
+            const u32 I = I0 + (u32)t;
+            u64 z[R2];
+            const u64 *row = lds + t * TS + ka * R2;
+#pragma unroll
+            for (int rb = 0; rb < R2; rb++) z[rb] = row[lds_col<R2>(rb, ka)];
+            dft<R2>(z);
+            if (I < a.total_inst) {
+                const u32 b = I / a.ninst;
+                const u32 i = I - b * a.ninst;
+                const u32 s = (u32)(i % (u32)a.S);
+                const u32 m = (u32)(i / (u32)a.S);
+                u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
+#pragma unroll
+                for (int kb = 0; kb < R2; kb++) {
+                    u64 v = z[kb];
+                    if (a.flags & FLAG_TWIDDLE) {
+                        u64 e = (u64)m * (u64)(ka + R1 * kb);
+                        v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                    }
+                    if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                    dst[a.S * (u64)(R1 * kb)] = v;
+                }
+            }
+        }
+    }
+}
+
+template <int R1, int R2, int NT, int T>
+__global__ __launch_bounds__(NT) void ntt_pass_kernel(PassArgs a) {
+    __shared__ u64 lds[lds_words<R1, R2, T>()];
+    pass_body<R1, R2, NT, T>(a, blockIdx.x, lds);
+}
+
+// Grouped launch: descriptor j owns blocks [starts[j], starts[j+1]).  Used
+// to run the same pass variant of many transforms (of different lengths)
+// in one launch.
+template <int R1, int R2, int NT, int T>
+__global__ __launch_bounds__(NT) void ntt_pass_grouped_kernel(const PassArgs *__restrict__ descs,
+                                                              const u32 *__restrict__ starts, u32 nd) {
+    __shared__ u64 lds[lds_words<R1, R2, T>()];
+    const u32 b = blockIdx.x;
+    u32 lo = 0, hi = nd;  // find the last j with starts[j] <= b
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) / 2;
+        if (starts[mid] <= b) lo = mid; else hi = mid;
+    }
+    const PassArgs a = descs[lo];
+    pass_body<R1, R2, NT, T>(a, b - starts[lo], lds);
+}
+
+template <int R1, int R2, int NT, int T>
+static void launch_pass(const PassArgs &a, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((ntt_pass_kernel<R1, R2, NT, T>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <int R1, int R2, int NT, int T>
+static void launch_grouped(const PassArgs *descs, const u32 *starts, u32 nd, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((ntt_pass_grouped_kernel<R1, R2, NT, T>), dim3(grid), dim3(NT), 0, s, descs, starts, nd);
+}
+
+// T: enough instances that both sub-steps have >= NT tasks, within 136 KiB LDS.
+constexpr int pick_T(int R1, int R2, int NT) {
+    int mn = R1 < R2 ? R1 : R2;
+    if (R2 == 1) return NT;
+    int t = (NT + mn - 1) / mn;
+    while (t > 1 && (size_t)t * (R1 * R2 + 1) * 8 > 136 * 1024) t--;
+    return t;
+}
+
+#define GL_VARIANT(R1, R2)                                                              \
+    {R1, R2, pick_T(R1, R2, 256), 256,                                                  \
+     (R2 > 1 ? (size_t)pick_T(R1, R2, 256) * (R1 * R2 + 1) * 8 : 8),                    \
+     &launch_pass<R1, R2, 256, pick_T(R1, R2, 256)>,                                     \
+     &launch_grouped<R1, R2, 256, pick_T(R1, R2, 256)>}
+
+static const PassVariant kVariants[] = {
+    // powers of two
+    GL_VARIANT(2, 1), GL_VARIANT(4, 1), GL_VARIANT(8, 1), GL_VARIANT(16, 1),
+    GL_VARIANT(32, 1), GL_VARIANT(64, 1),
+    GL_VARIANT(2, 2), GL_VARIANT(4, 2), GL_VARIANT(4, 4), GL_VARIANT(8, 4),
+    GL_VARIANT(8, 8), GL_VARIANT(16, 8), GL_VARIANT(16, 16), GL_VARIANT(32, 16),
+    GL_VARIANT(32, 32), GL_VARIANT(64, 32), GL_VARIANT(64, 64),
+    // odd factors 3, 5, 15, 17 as their own sub-step
+    GL_VARIANT(3, 1), GL_VARIANT(5, 1), GL_VARIANT(15, 1), GL_VARIANT(17, 1),
+    GL_VARIANT(64, 3), GL_VARIANT(64, 5), GL_VARIANT(64, 15), GL_VARIANT(64, 17),
+    GL_VARIANT(32, 3), GL_VARIANT(32, 5), GL_VARIANT(32, 15), GL_VARIANT(32, 17),
+    GL_VARIANT(16, 3), GL_VARIANT(16, 5), GL_VARIANT(16, 15), GL_VARIANT(16, 17),
+    GL_VARIANT(8, 3), GL_VARIANT(8, 5), GL_VARIANT(8, 15), GL_VARIANT(8, 17),
+    GL_VARIANT(4, 3), GL_VARIANT(4, 5), GL_VARIANT(4, 15), GL_VARIANT(4, 17),
+    GL_VARIANT(2, 3), GL_VARIANT(2, 5), GL_VARIANT(2, 15), GL_VARIANT(2, 17),
+};
+
+const PassVariant *pass_variants(int *count) {
+    *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+    return kVariants;
+}
+
+// ---- elementwise kernels ----
+
+__device__ __forceinline__ u64 splitmix64(u64 &state) {
+    u64 z = (state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Word i: first splitmix64 output < p of the stream seeded at
+// seed + i * 0x9e3779b97f4a7c15 (the same map the tests' generator uses).
+__global__ void fill_random_kernel(u64 *v, u64 n, u64 seed) {
+    for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+        u64 st = seed + i * 0x9e3779b97f4a7c15ull, x;
+        do x = splitmix64(st); while (x >= P);
+        v[i] = x;
+    }
+}
+
+void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s) {
+    unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(fill_random_kernel, dim3(grid), dim3(256), 0, s, v, n, seed);
+}
+
+// intt pre-step for items handled without a fused first pass:
+// v = scale * v[(n - j) mod n] per item, in place (pairs swapped by one thread).
+__global__ void scale_reverse_kernel(u64 *v, u64 n, u64 count, u64 scale) {
+    u64 half = n / 2 + 1;  // j in [0, half) pairs with n - j
+    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < half * count;
+         g += (u64)gridDim.x * blockDim.x) {
+        u64 b = g / half, j = g % half;
+        u64 *x = v + b * n;
+        u64 k = j ? n - j : 0;
+        if (k < j) continue;
+        u64 a = x[j], c = x[k];
+        x[j] = mul(c, scale);
+        if (k != j) x[k] = mul(a, scale);
+    }
+}
+
+void launch_scale_reverse(u64 *v, u64 n, u64 count, u64 scale, hipStream_t s) {
+    u64 work = (n / 2 + 1) * count;
+    unsigned grid = (unsigned)((work + 255) / 256 < 65536 ? (work + 255) / 256 : 65536);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(scale_reverse_kernel, dim3(grid), dim3(256), 0, s, v, n, count, scale);
+}
+
+// Coset extension for the commit: ext[r][c] = coef[r][c] * g^r for r < rows,
+// zero for rows <= r < blowup*rows (g^r = lo[r & mask] * hi[r >> lo_bits]).
+__global__ void coset_extend_kernel(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
+                                    const u64 *lo, const u64 *hi, u32 lo_bits) {
+    u64 total = rows * blowup * cols;
+    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        u64 r = g / cols;
+        u64 v = 0;
+        if (r < rows) v = mul(coef[g], mul(lo[r & ((1ull << lo_bits) - 1)], hi[r >> lo_bits]));
+        ext[g] = v;
+    }
+}
+
+void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
+                         const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s) {
+    u64 total = rows * blowup * cols;
+    unsigned grid = (unsigned)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(coset_extend_kernel, dim3(grid), dim3(256), 0, s, coef, ext, rows, cols,
+                       blowup, pw_lo, pw_hi, lo_bits);
+}
+
+}  // namespace gl

@@ -1,0 +1,434 @@
+// Host runtime for the gfx950 Goldilocks library: errors, device pool,
+// twiddle tables, the pass planner and plan execution.
+#include "gl_runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "gl_field.hpp"
+
+namespace gl {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string t_error;
+
+int set_error(int code, const std::string &msg) {
+    t_error = msg;
+    return code;
+}
+
+int hip_error(hipError_t e, const char *what) {
+    int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GL_E_NOMEM : GL_E_DEVICE;
+    return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const char *last_error_cstr() { return t_error.c_str(); }
+
+static std::once_flag g_init_once;
+static int g_init_rc = GL_OK;
+static std::string g_init_msg;
+
+int ensure_init() {
+    std::call_once(g_init_once, [] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) {
+            g_init_rc = GL_E_DEVICE;
+            g_init_msg = "no HIP device available";
+            return;
+        }
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+            g_init_rc = GL_E_DEVICE;
+            g_init_msg = "hipGetDeviceProperties failed";
+            return;
+        }
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            g_init_rc = GL_E_DEVICE;
+            g_init_msg = std::string("library is built for gfx950, device is ") + prop.gcnArchName;
+        }
+    });
+    if (g_init_rc != GL_OK) return set_error(g_init_rc, g_init_msg);
+    return GL_OK;
+}
+
+// ---------------------------------------------------------------- pool
+static std::mutex g_pool_mu;
+static std::multimap<size_t, void *> g_pool_free;
+static std::map<void *, size_t> g_pool_size;
+
+static size_t round_size(size_t b) {
+    const size_t MB = 1 << 20;
+    if (b >= MB) return (b + MB - 1) / MB * MB;
+    return (b + 4095) / 4096 * 4096;
+}
+
+void *pool_alloc(size_t bytes) {
+    size_t sz = round_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pool_free.lower_bound(sz);
+        if (it != g_pool_free.end() && it->first <= sz + sz / 4) {
+            void *p = it->second;
+            g_pool_free.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim();
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_size[p] = sz;
+    return p;
+}
+
+void pool_free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_size.find(p);
+    if (it == g_pool_size.end()) return;
+    g_pool_free.emplace(it->second, p);
+}
+
+void pool_trim() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (g_pool_free.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto &kv : g_pool_free) {
+        g_pool_size.erase(kv.second);
+        (void)hipFree(kv.second);
+    }
+    g_pool_free.clear();
+}
+
+struct ThreadStream {
+    hipStream_t s = nullptr;
+    ~ThreadStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+static thread_local ThreadStream t_stream;
+
+hipStream_t thread_stream() {
+    if (!t_stream.s) {
+        if (hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            t_stream.s = nullptr;
+        }
+    }
+    return t_stream.s;
+}
+
+static std::mutex g_scratch_mu;
+static std::map<std::pair<hipStream_t, int>, std::pair<void *, size_t>> g_scratch;
+
+u64 *stream_scratch(hipStream_t s, size_t words, int slot) {
+    if (words == 0) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto &e = g_scratch[std::make_pair(s, slot)];
+    if (e.second >= words * 8) return (u64 *)e.first;
+    if (e.first) {
+        (void)hipStreamSynchronize(s);
+        pool_free(e.first);
+        e.first = nullptr;
+        e.second = 0;
+    }
+    void *p = pool_alloc(words * 8);
+    if (!p) return nullptr;
+    e.first = p;
+    e.second = words * 8;
+    return (u64 *)p;
+}
+
+static void release_scratch() {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    for (auto &kv : g_scratch)
+        if (kv.second.first) pool_free(kv.second.first);
+    g_scratch.clear();
+}
+
+// ---------------------------------------------------------------- twiddles
+static std::mutex g_tw_mu;
+static std::map<std::pair<u64, u64>, PowTable> g_pow_tables;  // (base, count)
+static std::map<u64, const u64 *> g_inner_tables;
+static std::vector<void *> g_tw_allocs;
+
+static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+static const u64 *upload(const std::vector<u64> &h) {
+    void *d = nullptr;
+    if (hipMalloc(&d, h.size() * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_tw_allocs.push_back(d);
+    return (const u64 *)d;
+}
+
+PowTable power_table(u64 base, u64 count) {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(base, count);
+    auto it = g_pow_tables.find(key);
+    if (it != g_pow_tables.end()) return it->second;
+    u32 lo_bits = (u32)((bit_length(count > 1 ? count - 1 : 1) + 1) / 2);
+    u64 L = 1ull << lo_bits;
+    u64 H = (count + L - 1) / L;
+    std::vector<u64> lo(L), hi(H ? H : 1);
+    u64 w = 1;
+    for (u64 e = 0; e < L; e++) {
+        lo[e] = w;
+        w = mul(w, base);
+    }
+    u64 wl = w, v = 1;  // wl = base^L
+    for (u64 j = 0; j < hi.size(); j++) {
+        hi[j] = v;
+        v = mul(v, wl);
+    }
+    PowTable t;
+    t.lo = upload(lo);
+    t.hi = upload(hi);
+    t.lo_bits = lo_bits;
+    if (t.lo && t.hi) g_pow_tables[key] = t;
+    return t;
+}
+
+PowTable root_table(u64 M) { return power_table(root(M), M); }
+
+const u64 *inner_table(u64 R) {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_inner_tables.find(R);
+    if (it != g_inner_tables.end()) return it->second;
+    std::vector<u64> t(R);
+    u64 w = root(R), v = 1;
+    for (u64 e = 0; e < R; e++) {
+        t[e] = v;
+        v = mul(v, w);
+    }
+    const u64 *d = upload(t);
+    if (d) g_inner_tables[R] = d;
+    return d;
+}
+
+// ---------------------------------------------------------------- planner
+static std::mutex g_plan_mu;
+static std::map<u64, std::unique_ptr<Plan>> g_plans;
+
+static const PassVariant *find_variant(int R1, int R2) {
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    for (int i = 0; i < cnt; i++)
+        if (vs[i].R1 == R1 && vs[i].R2 == R2) return &vs[i];
+    return nullptr;
+}
+
+// Pass with `bits` power-of-two bits and odd codelet g (g = 1: none).
+static const PassVariant *variant_for(int bits, int g) {
+    if (g == 1) {
+        if (bits < 1 || bits > 12) return nullptr;
+        return find_variant(1 << ((bits + 1) / 2), 1 << (bits / 2));
+    }
+    if (bits == 0) return find_variant(g, 1);
+    if (bits > 6) return nullptr;
+    return find_variant(1 << bits, g);
+}
+
+static bool build_plan(u64 n, Plan &plan) {
+    plan.n = n;
+    plan.passes.clear();
+    if (n == 1) return true;
+    int k = __builtin_ctzll(n);
+    u64 f = n >> k;
+    std::vector<int> primes;
+    for (u64 q : {3ull, 5ull, 17ull})
+        if (f % q == 0) {
+            primes.push_back((int)q);
+            f /= q;
+        }
+    if (f != 1) return false;  // 257 / 65537 factors: not scheduled by this planner
+    // Candidate groupings of the odd primes into single-codelet groups.
+    std::vector<std::vector<int>> groupings;
+    bool has3 = std::count(primes.begin(), primes.end(), 3) > 0;
+    bool has5 = std::count(primes.begin(), primes.end(), 5) > 0;
+    bool has17 = std::count(primes.begin(), primes.end(), 17) > 0;
+    {
+        std::vector<int> g;
+        if (has3) g.push_back(3);
+        if (has5) g.push_back(5);
+        if (has17) g.push_back(17);
+        groupings.push_back(g);
+        if (has3 && has5) {
+            std::vector<int> h{15};
+            if (has17) h.push_back(17);
+            groupings.push_back(h);
+        }
+    }
+    double best_score = 1e30;
+    std::vector<std::pair<int, int>> best;  // (bits, g)
+    for (int P = 1; P <= 4 && best.empty(); P++) {
+        for (auto &grp : groupings) {
+            if ((int)grp.size() > P) continue;
+            std::vector<int> g(P, 1), bits(P, 0), cap(P, 12), lo(P, 1);
+            for (size_t i = 0; i < grp.size(); i++) {
+                g[i] = grp[i];
+                cap[i] = 6;
+                lo[i] = 0;
+            }
+            int need = k;
+            for (int i = 0; i < P; i++) {
+                bits[i] = lo[i];
+                need -= lo[i];
+            }
+            int total_cap = 0;
+            for (int i = 0; i < P; i++) total_cap += cap[i] - lo[i];
+            if (need < 0 || need > total_cap) continue;
+            while (need > 0) {
+                int bi = -1;
+                double bs = 1e30;
+                for (int i = 0; i < P; i++) {
+                    if (bits[i] >= cap[i]) continue;
+                    double sz = bits[i] + std::log2((double)g[i]);
+                    if (sz < bs) {
+                        bs = sz;
+                        bi = i;
+                    }
+                }
+                bits[bi]++;
+                need--;
+            }
+            bool ok = true;
+            double mx = 0;
+            for (int i = 0; i < P; i++) {
+                if (!variant_for(bits[i], g[i])) ok = false;
+                mx = std::max(mx, bits[i] + std::log2((double)g[i]));
+            }
+            if (!ok) continue;
+            if (mx < best_score) {
+                best_score = mx;
+                best.clear();
+                for (int i = 0; i < P; i++) best.emplace_back(bits[i], g[i]);
+            }
+        }
+    }
+    if (best.empty()) return false;
+    for (auto &bg : best) {
+        const PassVariant *v = variant_for(bg.first, bg.second);
+        plan.passes.push_back({v, (u64)v->R1 * (u64)v->R2});
+    }
+    return true;
+}
+
+int get_plan(u64 n, const Plan **out) {
+    if (n == 0 || (P - 1) % n != 0)
+        return set_error(GL_E_SIZE, std::to_string(n) +
+                                        " is not a supported NTT size (does not divide the multiplicative order)");
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(n);
+    if (it == g_plans.end()) {
+        std::unique_ptr<Plan> p(new Plan());
+        if (!build_plan(n, *p))
+            return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
+        it = g_plans.emplace(n, std::move(p)).first;
+    }
+    *out = it->second.get();
+    return GL_OK;
+}
+
+size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride) {
+    return plan.passes.size() > 1 ? (size_t)(count * bstride) : 0;
+}
+
+int PassCursor::next(PassArgs &a, unsigned &grid) {
+    const size_t np = plan->passes.size();
+    const u64 n = plan->n;
+    const PassPlan &pp = plan->passes[p];
+    const u64 R = pp.R, Mp = M / R;
+    const bool last = p + 1 == np;
+    u64 *dst = last ? data : (src == data ? tmp : data);
+    std::memset(&a, 0, sizeof a);
+    a.in = src;
+    a.out = dst;
+    a.bstride = bstride;
+    a.S = S;
+    a.SMp = S * Mp;
+    a.n_rev = n;
+    u64 ninst = S * Mp, total = count * ninst;
+    if (total >= (1ull << 32) || S >= (1ull << 32))
+        return set_error(GL_E_UNSUPPORTED, "batch too large for one launch");
+    a.ninst = (u32)ninst;
+    a.total_inst = (u32)total;
+    a.Mp = (u32)Mp;
+    if (pp.v->R2 > 1) {
+        a.tw_inner = inner_table(R);
+        if (!a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    }
+    if (Mp > 1) {
+        PowTable t = root_table(M);
+        if (!t.lo || !t.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+        a.tw_lo = t.lo;
+        a.tw_hi = t.hi;
+        a.lo_bits = t.lo_bits;
+        a.flags |= FLAG_TWIDDLE;
+    }
+    if (inverse && p == 0) a.flags |= FLAG_REVERSE;
+    if (inverse && last) {
+        a.flags |= FLAG_SCALE;
+        a.scale = inv(n % P);
+    }
+    grid = (unsigned)((total + pp.v->T - 1) / pp.v->T);
+    src = dst;
+    S *= R;
+    M = Mp;
+    p++;
+    return GL_OK;
+}
+
+int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
+             hipStream_t stream) {
+    if (plan.passes.empty() || count == 0) return GL_OK;
+    PassCursor c{&plan, data, tmp, S0, count, bstride, inverse};
+    c.begin();
+    while (!c.done()) {
+        const PassVariant *v = plan.passes[c.p].v;
+        PassArgs a;
+        unsigned grid = 0;
+        int rc = c.next(a, grid);
+        if (rc) return rc;
+        v->launch(a, grid, stream);
+        GL_HIP(hipGetLastError());
+    }
+    return GL_OK;
+}
+
+void clear_caches() {
+    (void)hipDeviceSynchronize();
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_plans.clear();
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tw_mu);
+        for (void *p : g_tw_allocs) (void)hipFree(p);
+        g_tw_allocs.clear();
+        g_pow_tables.clear();
+        g_inner_tables.clear();
+    }
+    release_scratch();
+    pool_trim();
+}
+
+}  // namespace gl

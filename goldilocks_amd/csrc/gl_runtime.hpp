@@ -1,0 +1,100 @@
+// Host runtime: error reporting, device memory pool, twiddle tables, the
+// NTT planner and plan execution.  Shared by the C ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gl_internal.hpp"
+#include "../../include/goldilocks_gpu.h"
+
+namespace gl {
+
+// ---- errors ----
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+#define GL_HIP(call)                                              \
+    do {                                                          \
+        hipError_t _e = (call);                                   \
+        if (_e != hipSuccess) return ::gl::hip_error(_e, #call); \
+    } while (0)
+
+int ensure_init();
+
+// ---- device memory pool (buffers are reused by size class) ----
+void *pool_alloc(size_t bytes);
+void pool_free(void *p);
+void pool_trim();
+
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) : p(bytes ? pool_alloc(bytes) : nullptr) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { if (p) pool_free(p); }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+hipStream_t thread_stream();
+
+// Scratch words owned by a stream: work enqueued on one stream runs in order,
+// so successive calls on that stream can share the buffer without waiting.
+// Growing it waits for the stream before releasing the old buffer.
+u64 *stream_scratch(hipStream_t s, size_t words, int slot = 0);
+
+// ---- twiddles ----
+struct PowTable {  // w^e = lo[e & (2^lo_bits - 1)] * hi[e >> lo_bits], e < order
+    const u64 *lo = nullptr, *hi = nullptr;
+    u32 lo_bits = 0;
+};
+// Powers of w_M (root of order M).
+PowTable root_table(u64 M);
+// Powers of an arbitrary base g for exponents < count (e.g. coset shifts).
+PowTable power_table(u64 base, u64 count);
+// w_R^e for e < R (inner twiddles of a two-sub-step pass).
+const u64 *inner_table(u64 R);
+
+// ---- planning ----
+struct PassPlan {
+    const PassVariant *v;
+    u64 R;
+};
+struct Plan {
+    u64 n = 0;
+    std::vector<PassPlan> passes;
+};
+// Returns GL_OK and a pass list for length n, GL_E_SIZE or GL_E_UNSUPPORTED.
+int get_plan(u64 n, const Plan **out);
+
+// Pass p of a plan as launch arguments.  `src_is_data` tracks the ping-pong
+// state: pass 0 reads data; non-final passes alternate data <-> tmp; the
+// final pass writes data (in place when it also reads data).
+struct PassCursor {
+    const Plan *plan;
+    u64 *data, *tmp;
+    u64 S0, count, bstride;
+    bool inverse;
+    // evolving state
+    size_t p = 0;
+    u64 S = 0, M = 0;
+    u64 *src = nullptr;
+    void begin() { p = 0; S = S0; M = plan->n; src = data; }
+    bool done() const { return p >= plan->passes.size(); }
+    // Fills args/grid for the current pass and advances; returns GL_OK or an error.
+    int next(PassArgs &a, unsigned &grid);
+};
+
+// Run `count` items of bstride words, each holding S0 interleaved transforms
+// of length n (S0 = 1: contiguous transforms; S0 = cols: matrix columns).
+// tmp must hold count*bstride words when the plan has more than one pass.
+int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
+             hipStream_t stream);
+size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
+
+void clear_caches();
+
+}  // namespace gl

@@ -1,0 +1,136 @@
+/*
+ * goldilocks_gpu.h - C ABI of the MI355X (gfx950) Goldilocks NTT and K12
+ * Merkle library (libgoldilocks_amd.so).
+ *
+ * Every pointer is a plain host or device address; every "Field" word is the
+ * raw u64 the reference keeps in a `&mut [Field]` (Montgomery form, canonical
+ * in [0, p), p = 2^64 - 2^32 + 1; ntt/src/field/mod.rs:8-12).  Results are
+ * bit-exact with the reference's CPU path on every output word and hash.
+ *
+ * Errors: functions return 0 (GL_OK) or a GL_E_* code and never abort the
+ * host; gl_last_error() gives a message for the calling thread.  Where the
+ * reference panics (ntt/src/ntt/mod.rs:55-58, pcs/src/k12/generic.rs:65-66),
+ * these return GL_E_SIZE / GL_E_ARG.
+ *
+ * Threading: all entry points may be called from several host threads; the
+ * plan and twiddle caches are mutex guarded and host-buffer calls use a
+ * stream per calling thread.  Device-buffer calls ("_dev") enqueue on the
+ * caller's stream (NULL = the default stream) and return without
+ * synchronizing unless stated otherwise.
+ */
+#ifndef GOLDILOCKS_GPU_H
+#define GOLDILOCKS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GL_OK 0
+#define GL_E_SIZE 1        /* length does not divide p - 1 (or is 0 for a forward transform) */
+#define GL_E_NOMEM 2       /* device allocation failed or an HBM budget was exceeded */
+#define GL_E_DEVICE 3      /* HIP runtime error */
+#define GL_E_ARG 4         /* invalid argument (null pointer, short buffer, K12 input > 166 B) */
+#define GL_E_UNSUPPORTED 5 /* legal size the device planner cannot schedule */
+#define GL_E_BUSY 6        /* server: no slot / budget right now (retry) */
+
+/* Message for the last error on the calling thread ("" if none). */
+const char *gl_last_error(void);
+
+/* Select and initialise a device (optional; the first call initialises the
+ * current device).  Returns GL_E_DEVICE when no gfx950 device is usable. */
+int gl_init(int device);
+
+/* ntt/src/divisors.rs:21-23: n > 0 and n | p - 1. */
+int gl_is_divisor(uint64_t n);
+
+/* ---- NTT, host buffers (copied to HBM and back; synchronous) ------------- */
+
+/* Forward transform in place.  Replaces goldilocks_ntt::ntt::ntt
+ * (ntt/src/ntt/mod.rs:98-102). */
+int gl_ntt(uint64_t *v, size_t n);
+
+/* Inverse transform in place: scale by 1/n, reverse v[1..], forward.
+ * n <= 1 is a no-op.  Replaces ntt::intt (ntt/src/ntt/mod.rs:104-120). */
+int gl_intt(uint64_t *v, size_t n);
+
+/* `count` transforms of length n stored back to back.  Replaces
+ * strategy(n).ntt(values) with values.len() == n * count
+ * (ntt/src/ntt/mod.rs:48-78, trait Ntt :25-31); inverse != 0 runs intt on
+ * each item. */
+int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse);
+
+/* k transforms of individual lengths n[i] at v[i] (mixed sizes and radices),
+ * scheduled as one batched launch sequence.  No reference counterpart beyond
+ * looping ntt::ntt over the slices; results equal that loop. */
+int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k, int inverse);
+
+/* ---- NTT, device buffers (words already in HBM) ------------------------- */
+
+/* As gl_ntt_batch on a device buffer of n*count words, enqueued on `stream`. */
+int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream);
+
+/* Column transforms of a row-major (n x cols) matrix: column c is words
+ * c, c + cols, c + 2*cols, ...; `count` such matrices back to back. */
+int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int inverse, void *stream);
+
+/* Mixed-size batch on device buffers d[i] of n[i] words. */
+int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, int inverse, void *stream);
+
+/* The same batch, one launch sequence per transform (the unbatched
+ * baseline the batched path is measured against). */
+int gl_ntt_mixed_dev_unbatched(uint64_t *const *d, const size_t *n, size_t k, int inverse,
+                               void *stream);
+
+/* Seeded uniform words in [0, p): word i is the first splitmix64 output
+ * below p of the stream seeded at seed + i * 0x9e3779b97f4a7c15. */
+int gl_fill_random_dev(uint64_t *d, size_t n, uint64_t seed, void *stream);
+
+/* Describe the device schedule for length n: pass i runs a radix
+ * r1[i]*r2[i] kernel variant with t[i] instances per workgroup.  Fills up to
+ * cap entries, *npasses = pass count.  (Planner introspection; the
+ * reference's counterpart is the plan tree strategy(n) builds.) */
+int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses);
+
+/* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46). */
+void gl_clear_cache(void);
+
+/* ---- K12 / Merkle (pcs/src/k12) ----------------------------------------- */
+
+/* k12::<H>(input) for len <= 166, outlen <= 168 (pcs/src/k12/generic.rs:62-84). */
+int gl_k12(const uint8_t *in, size_t len, uint8_t *out, size_t outlen);
+
+/* next_layer_size (pcs/src/k12/mod.rs:33-35): ceil(len / 160) * 32. */
+size_t gl_next_layer_size(size_t len);
+
+/* Total bytes of all layers merkle_tree(len) returns. */
+size_t gl_merkle_tree_size(size_t len);
+
+/* process_layer (pcs/src/k12/generic.rs:86-94, xkcp.rs:5-14):
+ * out[32i..32i+32] = K12(in[160i .. min(160i+160, len)]). */
+int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out);
+int gl_k12_layer_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, void *stream);
+
+/* merkle_tree (pcs/src/k12/mod.rs:41-60): every layer, concatenated in
+ * order (sizes follow from gl_next_layer_size), root last.  *out_len gets
+ * the total; GL_E_ARG if cap is smaller (then *out_len = required size). */
+int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size_t *out_len);
+int gl_merkle_tree_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
+                       size_t *out_len, void *stream);
+
+/* ---- Commit (no single reference function; composed from ntt::intt,
+ * coefficient scaling by GENERATOR^i, ntt::ntt and merkle_tree over the raw
+ * bytes as bench/src/main.rs:179-183 hashes them) ---------------------------
+ * m: row-major rows x cols words.  lde_out: (blowup*rows) x cols row-major
+ * coset low-degree extension; layers_out: merkle_tree of its bytes. */
+int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
+              uint8_t *layers_out, size_t cap, size_t *out_len);
+int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup,
+                  uint64_t *d_lde, uint8_t *d_layers, size_t cap, size_t *out_len, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,45 @@
+"""GPU commit (column coset LDE + K12 Merkle) parity against the oracle."""
+import numpy as np
+import pytest
+
+from gpu_util import from_torch, note, to_torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gl():
+    import goldilocks_amd as g
+    assert g.lib().gl_init(-1) == 0, g.lib().gl_last_error()
+    return g
+
+
+@pytest.mark.parametrize("rows,cols,blowup", [(16, 4, 2), (8, 3, 2), (1024, 64, 2), (4096, 7, 4),
+                                              (2**14, 16, 2), (240, 5, 2), (3 * 2**10, 2, 2)])
+def test_commit_host_vs_oracle(gl, oracle, rows, cols, blowup):
+    note(f"commit {rows}x{cols} blowup {blowup}")
+    m = oracle.fill_random(rows * cols, rows * 31 + cols).reshape(rows, cols)
+    lde, layers = gl.commit(m, blowup)
+    lde_o, layers_o = oracle.commit(m, blowup)
+    assert np.array_equal(lde, lde_o)
+    assert layers == layers_o
+
+
+def test_commit_device_medium(gl, oracle):
+    """2^16 x 64 with blowup 2 (a 1/16-height slice of the 2^20 x 64 config)."""
+    import torch
+    rows, cols = 2**16, 64
+    note(f"commit device {rows}x{cols}")
+    m = torch.empty((rows, cols), dtype=torch.int64, device="cuda")
+    gl.fill_random(m, 11)
+    lde, layers = gl.commit(m, 2)
+    torch.cuda.synchronize()
+    hm = from_torch(m).reshape(rows, cols)
+    lde_o, layers_o = oracle.commit(hm, 2)
+    assert np.array_equal(from_torch(lde).reshape(2 * rows, cols), lde_o)
+    assert layers.cpu().numpy().tobytes() == b"".join(layers_o)
+
+
+def test_commit_errors(gl):
+    with pytest.raises(ValueError):
+        gl.commit(np.zeros((7, 2), dtype=np.uint64), 2)
