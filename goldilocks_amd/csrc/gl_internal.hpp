@@ -54,6 +54,9 @@ struct PassVariant {
 // All compiled pass variants (gl_ntt_kernels.hip).
 const PassVariant *pass_variants(int *count);
 
+// Radix-257 Rader pass (gl_rader.hip).
+const PassVariant *rader257_variant();
+
 // Elementwise kernels (gl_ntt_kernels.hip).
 void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
 void launch_scale_reverse(u64 *v, u64 n, u64 count, u64 scale, hipStream_t s);

@@ -162,6 +162,7 @@ static std::mutex g_tw_mu;
 static std::map<std::pair<u64, u64>, PowTable> g_pow_tables;  // (base, count)
 static std::map<u64, const u64 *> g_inner_tables;
 static std::vector<void *> g_tw_allocs;
+static const u64 *g_rader257 = nullptr;
 
 static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
@@ -224,6 +225,40 @@ const u64 *inner_table(u64 R) {
     return d;
 }
 
+// Tables for the radix-257 Rader pass: w_256^e, the convolution spectrum
+// DFT_256(w_257^(3^q) / 256), 86^q mod 257 and 3^q mod 257 (rader.rs:19-57).
+const u64 *rader257_table() {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (g_rader257) return g_rader257;
+    std::vector<u64> t(1024);
+    u64 w256 = root(256), w257 = root(257), s256 = inv(256);
+    u64 v = 1;
+    for (int e = 0; e < 256; e++) {
+        t[e] = v;
+        v = mul(v, w256);
+    }
+    std::vector<u64> c(256);
+    u64 gq = 1;
+    for (int q = 0; q < 256; q++) {
+        c[q] = mul(pow(w257, gq), s256);
+        gq = gq * 3 % 257;
+    }
+    for (int k = 0; k < 256; k++) {
+        u64 acc = 0;
+        for (int q = 0; q < 256; q++) acc = add(acc, mul(c[q], t[(q * k) & 255]));
+        t[256 + k] = acc;
+    }
+    u64 pk = 1, pi = 1;
+    for (int q = 0; q < 256; q++) {
+        t[512 + q] = pk;
+        t[768 + q] = pi;
+        pk = pk * 86 % 257;
+        pi = pi * 3 % 257;
+    }
+    g_rader257 = upload(t);
+    return g_rader257;
+}
+
 // ---------------------------------------------------------------- planner
 static std::mutex g_plan_mu;
 static std::map<u64, std::unique_ptr<Plan>> g_plans;
@@ -253,13 +288,17 @@ static bool build_plan(u64 n, Plan &plan) {
     if (n == 1) return true;
     int k = __builtin_ctzll(n);
     u64 f = n >> k;
+    const bool rader257 = f % 257 == 0;
+    if (rader257) f /= 257;
     std::vector<int> primes;
     for (u64 q : {3ull, 5ull, 17ull})
         if (f % q == 0) {
             primes.push_back((int)q);
             f /= q;
         }
-    if (f != 1) return false;  // 257 / 65537 factors: not scheduled by this planner
+    if (f != 1) return false;  // a 65537 factor: not scheduled by this planner
+    if (rader257) plan.passes.push_back({rader257_variant(), 257});
+    if (k == 0 && primes.empty()) return true;
     // Candidate groupings of the odd primes into single-codelet groups.
     std::vector<std::vector<int>> groupings;
     bool has3 = std::count(primes.begin(), primes.end(), 3) > 0;
@@ -372,8 +411,8 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     a.ninst = (u32)ninst;
     a.total_inst = (u32)total;
     a.Mp = (u32)Mp;
-    if (pp.v->R2 > 1) {
-        a.tw_inner = inner_table(R);
+    if (pp.v->R2 > 1 || pp.v->R1 == 257) {
+        a.tw_inner = pp.v->R1 == 257 ? rader257_table() : inner_table(R);
         if (!a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     }
     if (Mp > 1) {
@@ -426,6 +465,7 @@ void clear_caches() {
         g_tw_allocs.clear();
         g_pow_tables.clear();
         g_inner_tables.clear();
+        g_rader257 = nullptr;
     }
     release_scratch();
     pool_trim();

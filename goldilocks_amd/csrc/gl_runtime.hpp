@@ -57,6 +57,8 @@ PowTable root_table(u64 M);
 PowTable power_table(u64 base, u64 count);
 // w_R^e for e < R (inner twiddles of a two-sub-step pass).
 const u64 *inner_table(u64 R);
+// Tables of the radix-257 Rader pass (see gl_rader.hip).
+const u64 *rader257_table();
 
 // ---- planning ----
 struct PassPlan {

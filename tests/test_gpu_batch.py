@@ -17,7 +17,7 @@ def gl():
     return g
 
 
-def mixed_sizes(count, lo, hi, seed, factors=(1, 3, 5, 15, 17)):
+def mixed_sizes(count, lo, hi, seed, factors=(1, 3, 5, 15, 17, 257)):
     rng = random.Random(seed)
     out = []
     while len(out) < count:
