@@ -63,6 +63,42 @@ int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
 
 }  // namespace
 
+namespace gl {
+
+int commit_check(size_t rows, size_t cols, unsigned blowup) {
+    const u64 ext = (u64)rows * blowup;
+    if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
+    if (!((P - 1) % rows == 0) || !((P - 1) % ext == 0))
+        return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
+    return GL_OK;
+}
+
+size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup) {
+    return tree_size((u64)rows * blowup * cols * 8);
+}
+
+// Enqueue the whole commit on stream s (no host synchronization):
+// coef (rows*cols words) and tmp (blowup*rows*cols words) are scratch.
+int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
+                   u64 *coef, u64 *tmp, hipStream_t s) {
+    const u64 ext = (u64)rows * blowup;
+    const Plan *prow = nullptr, *pext = nullptr;
+    int rc;
+    if ((rc = get_plan(rows, &prow))) return rc;
+    if ((rc = get_plan(ext, &pext))) return rc;
+    GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
+    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s))) return rc;
+    PowTable g = power_table(2717, rows);
+    if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
+    GL_HIP(hipGetLastError());
+    if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s))) return rc;
+    size_t need = tree_size(ext * cols * 8);
+    return merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, need, nullptr, s);
+}
+
+}  // namespace gl
+
 extern "C" {
 
 const char *gl_last_error(void) { return last_error_cstr(); }
@@ -213,30 +249,18 @@ int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup
     int rc = ensure_init();
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
+    if ((rc = commit_check(rows, cols, blowup))) return rc;
     const u64 ext = (u64)rows * blowup;
-    if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
-    if (!gl_is_divisor(rows) || !gl_is_divisor(ext))
-        return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
     size_t need = tree_size(ext * cols * 8);
     if (out_len) *out_len = need;
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
     if (!d_m || !d_lde || !d_layers) return set_error(GL_E_ARG, "null device pointer");
-    const Plan *prow = nullptr, *pext = nullptr;
-    if ((rc = get_plan(rows, &prow))) return rc;
-    if ((rc = get_plan(ext, &pext))) return rc;
-    // coefficients: column intt of a copy of the input
     DevBuf coef(rows * cols * 8);
     DevBuf tmp(ext * cols * 8);
     if (!coef.p || !tmp.p) return set_error(GL_E_NOMEM, "commit scratch allocation failed");
-    GL_HIP(hipMemcpyAsync(coef.p, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
-    if ((rc = run_plan(*prow, coef.as<u64>(), tmp.as<u64>(), cols, 1, rows * cols, true, s))) return rc;
-    PowTable g = power_table(2717, rows);
-    if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-    launch_coset_extend(coef.as<u64>(), d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
-    GL_HIP(hipGetLastError());
-    if ((rc = run_plan(*pext, d_lde, tmp.as<u64>(), cols, 1, ext * cols, false, s))) return rc;
-    if ((rc = merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, cap, nullptr, s)))
+    if ((rc = commit_enqueue(d_m, rows, cols, blowup, d_lde, d_layers, coef.as<u64>(), tmp.as<u64>(), s)))
         return rc;
+    // coef/tmp return to the pool on exit: finish the work first.
     GL_HIP(hipStreamSynchronize(s));
     return GL_OK;
 }
@@ -245,10 +269,8 @@ int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint
               uint8_t *layers_out, size_t cap, size_t *out_len) {
     int rc = ensure_init();
     if (rc) return rc;
+    if ((rc = commit_check(rows, cols, blowup))) return rc;
     const u64 ext = (u64)rows * blowup;
-    if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
-    if (!gl_is_divisor(rows) || !gl_is_divisor(ext))
-        return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
     size_t need = tree_size(ext * cols * 8);
     if (out_len) *out_len = need;
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");

@@ -99,4 +99,10 @@ size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
 
 void clear_caches();
 
+// Commit pieces shared by the C ABI and the server (gl_api.cpp).
+int commit_check(size_t rows, size_t cols, unsigned blowup);
+size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup);
+int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
+                   u64 *coef, u64 *tmp, hipStream_t s);
+
 }  // namespace gl

@@ -1,0 +1,339 @@
+// Continuous-batching request server for NTT and commit jobs.
+//
+// One scheduler thread owns `slots` non-blocking HIP streams.  Jobs queue in
+// FIFO order; the head job is admitted as soon as a stream is idle and its
+// device footprint fits in the unused HBM budget.  Admitted jobs are
+// enqueued whole on their stream (copies in, passes, copies out, an event),
+// so jobs that arrive while others run overlap with them on the device.
+// Completion is observed by polling the job's event; the job's buffers then
+// return to the device pool and its budget is released.  Plans and twiddle
+// tables are cached process-wide and stay resident across jobs.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "gl_runtime.hpp"
+
+using namespace gl;
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job {
+    gl_job spec;
+    uint64_t ticket = 0;
+    size_t footprint = 0;
+    double t_submit = 0, t_done = 0;
+    int status = GL_OK;
+    bool done = false;
+    // in flight
+    int slot = -1;
+    std::vector<void *> bufs;
+    hipEvent_t event = nullptr;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    std::shared_ptr<Job> job;
+};
+
+size_t ntt_footprint(const gl_job &j, const Plan *plan) {
+    size_t words = j.n * j.count;
+    size_t f = plan && plan->passes.size() > 1 ? words * 8 : 0;
+    if (!j.device) f += words * 8;
+    return f;
+}
+
+size_t commit_footprint(const gl_job &j) {
+    size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
+    size_t f = rows * cols * 8 + ext * cols * 8;  // coef + tmp
+    if (!j.device) f += rows * cols * 8 + ext * cols * 8 + commit_tree_size(rows, cols, j.blowup);
+    return f;
+}
+
+}  // namespace
+
+struct gl_server {
+    size_t budget = 0, in_use = 0, peak = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::map<uint64_t, std::shared_ptr<Job>> jobs;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    // stats
+    uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0;
+    std::vector<double> latencies;
+    double elements = 0, busy_ms = 0, busy_since = -1;
+    int inflight = 0;
+
+    void loop();
+    int start_job(Slot &slot, const std::shared_ptr<Job> &job);
+    void finish_job(Slot &slot, int status);
+};
+
+int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
+    const gl_job &j = job->spec;
+    hipStream_t s = slot.stream;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = bytes ? pool_alloc(bytes) : nullptr;
+        if (p) job->bufs.push_back(p);
+        return p;
+    };
+    int rc = GL_OK;
+    if (j.kind == GL_JOB_NTT || j.kind == GL_JOB_INTT) {
+        const bool inverse = j.kind == GL_JOB_INTT;
+        if (inverse && j.n <= 1) return GL_OK;
+        const Plan *plan = nullptr;
+        if ((rc = get_plan(j.n, &plan))) return rc;
+        size_t words = j.n * j.count;
+        u64 *data = j.data;
+        if (!j.device) {
+            data = (u64 *)alloc(words * 8);
+            if (!data) return set_error(GL_E_NOMEM, "server: allocation failed");
+            GL_HIP(hipMemcpyAsync(data, j.data, words * 8, hipMemcpyHostToDevice, s));
+        }
+        u64 *tmp = nullptr;
+        if (plan->passes.size() > 1) {
+            tmp = (u64 *)alloc(words * 8);
+            if (!tmp) return set_error(GL_E_NOMEM, "server: allocation failed");
+        }
+        if ((rc = run_plan(*plan, data, tmp, 1, j.count, j.n, inverse, s))) return rc;
+        if (!j.device) GL_HIP(hipMemcpyAsync(j.data, data, words * 8, hipMemcpyDeviceToHost, s));
+    } else {
+        size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
+        size_t tree = commit_tree_size(rows, cols, j.blowup);
+        const u64 *m = j.data;
+        u64 *lde = j.lde_out;
+        uint8_t *layers = j.layers_out;
+        if (!j.device) {
+            u64 *dm = (u64 *)alloc(rows * cols * 8);
+            lde = (u64 *)alloc(ext * cols * 8);
+            layers = (uint8_t *)alloc(tree + 256);
+            if (!dm || !lde || !layers) return set_error(GL_E_NOMEM, "server: allocation failed");
+            GL_HIP(hipMemcpyAsync(dm, j.data, rows * cols * 8, hipMemcpyHostToDevice, s));
+            m = dm;
+        }
+        u64 *coef = (u64 *)alloc(rows * cols * 8), *tmp = (u64 *)alloc(ext * cols * 8);
+        if (!coef || !tmp) return set_error(GL_E_NOMEM, "server: allocation failed");
+        if ((rc = commit_enqueue(m, rows, cols, j.blowup, lde, layers, coef, tmp, s))) return rc;
+        if (!j.device) {
+            GL_HIP(hipMemcpyAsync(j.lde_out, lde, ext * cols * 8, hipMemcpyDeviceToHost, s));
+            if (tree) GL_HIP(hipMemcpyAsync(j.layers_out, layers, tree, hipMemcpyDeviceToHost, s));
+        }
+    }
+    return GL_OK;
+}
+
+void gl_server::finish_job(Slot &slot, int status) {
+    std::shared_ptr<Job> job = slot.job;
+    slot.job.reset();
+    for (void *p : job->bufs) pool_free(p);
+    job->bufs.clear();
+    if (job->event) (void)hipEventDestroy(job->event);
+    job->event = nullptr;
+    job->t_done = now_ms();
+    job->status = status;
+    job->done = true;
+    in_use -= job->footprint;
+    inflight--;
+    if (inflight == 0 && busy_since >= 0) {
+        busy_ms += job->t_done - busy_since;
+        busy_since = -1;
+    }
+    if (status == GL_OK) {
+        completed++;
+        latencies.push_back(job->t_done - job->t_submit);
+        const gl_job &j = job->spec;
+        elements += j.kind == GL_JOB_COMMIT ? (double)j.n * j.blowup * j.count : (double)j.n * j.count;
+    } else {
+        failed++;
+    }
+}
+
+void gl_server::loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        // completions
+        bool progressed = false;
+        for (Slot &sl : slots) {
+            if (!sl.job) continue;
+            hipError_t q = hipEventQuery(sl.job->event);
+            if (q == hipErrorNotReady) continue;
+            finish_job(sl, q == hipSuccess ? GL_OK : hip_error(q, "server job"));
+            progressed = true;
+        }
+        if (progressed) cv_done.notify_all();
+        // admissions (FIFO head of line)
+        while (!queue.empty()) {
+            std::shared_ptr<Job> job = queue.front();
+            if (budget && in_use + job->footprint > budget) break;
+            Slot *free_slot = nullptr;
+            for (Slot &sl : slots)
+                if (!sl.job) {
+                    free_slot = &sl;
+                    break;
+                }
+            if (!free_slot) break;
+            queue.pop_front();
+            free_slot->job = job;
+            in_use += job->footprint;
+            peak = std::max(peak, in_use);
+            if (inflight++ == 0) busy_since = now_ms();
+            int rc = start_job(*free_slot, job);
+            hipError_t e = hipSuccess;
+            if (rc == GL_OK) e = hipEventCreateWithFlags(&job->event, hipEventDisableTiming);
+            if (rc == GL_OK && e == hipSuccess) e = hipEventRecord(job->event, free_slot->stream);
+            if (rc == GL_OK && e != hipSuccess) rc = hip_error(e, "server event");
+            if (rc != GL_OK) {
+                (void)hipStreamSynchronize(free_slot->stream);
+                finish_job(*free_slot, rc);
+                cv_done.notify_all();
+            }
+        }
+        if (stop && queue.empty() && inflight == 0) break;
+        // Sleep briefly: new submissions wake us; in-flight work is polled.
+        cv_work.wait_for(lk, std::chrono::microseconds(inflight ? 20 : 2000));
+    }
+}
+
+extern "C" {
+
+gl_server *gl_server_create(size_t hbm_budget_bytes, int slots) {
+    if (ensure_init() != GL_OK) return nullptr;
+    if (slots < 1 || slots > 16) {
+        set_error(GL_E_ARG, "server slots must be in [1, 16]");
+        return nullptr;
+    }
+    gl_server *srv = new gl_server();
+    srv->budget = hbm_budget_bytes;
+    srv->slots.resize(slots);
+    for (Slot &sl : srv->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            for (Slot &t : srv->slots)
+                if (t.stream) (void)hipStreamDestroy(t.stream);
+            delete srv;
+            set_error(GL_E_DEVICE, "server: stream creation failed");
+            return nullptr;
+        }
+    }
+    srv->worker = std::thread([srv] { srv->loop(); });
+    return srv;
+}
+
+int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket) {
+    if (!srv || !job) return set_error(GL_E_ARG, "null pointer");
+    auto j = std::make_shared<Job>();
+    j->spec = *job;
+    const gl_job &s = j->spec;
+    if (s.kind == GL_JOB_NTT || s.kind == GL_JOB_INTT) {
+        const Plan *plan = nullptr;
+        if (!(s.kind == GL_JOB_INTT && s.n <= 1)) {
+            int rc = get_plan(s.n, &plan);
+            if (rc) return rc;
+        }
+        if (s.count == 0 || (!s.data && s.n)) return set_error(GL_E_ARG, "server: empty or null NTT job");
+        j->footprint = ntt_footprint(s, plan);
+    } else if (s.kind == GL_JOB_COMMIT) {
+        int rc = commit_check(s.n, s.count, s.blowup);
+        if (rc) return rc;
+        if (!s.data || !s.lde_out || !s.layers_out) return set_error(GL_E_ARG, "server: null commit buffer");
+        j->footprint = commit_footprint(s);
+    } else {
+        return set_error(GL_E_ARG, "server: unknown job kind");
+    }
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->budget && j->footprint > srv->budget) {
+        srv->rejected++;
+        return set_error(GL_E_NOMEM, "server: job footprint exceeds the HBM budget");
+    }
+    j->ticket = srv->next_ticket++;
+    j->t_submit = now_ms();
+    srv->jobs[j->ticket] = j;
+    srv->queue.push_back(j);
+    srv->submitted++;
+    if (ticket) *ticket = j->ticket;
+    srv->cv_work.notify_one();
+    return GL_OK;
+}
+
+int gl_server_poll(gl_server *srv, uint64_t ticket, int *done) {
+    if (!srv) return set_error(GL_E_ARG, "null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(ticket);
+    if (it == srv->jobs.end()) return set_error(GL_E_ARG, "unknown ticket");
+    if (done) *done = it->second->done;
+    return it->second->done ? it->second->status : GL_OK;
+}
+
+int gl_server_wait(gl_server *srv, uint64_t ticket, double *latency_ms) {
+    if (!srv) return set_error(GL_E_ARG, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(ticket);
+    if (it == srv->jobs.end()) return set_error(GL_E_ARG, "unknown ticket");
+    std::shared_ptr<Job> j = it->second;
+    srv->cv_done.wait(lk, [&] { return j->done; });
+    if (latency_ms) *latency_ms = j->t_done - j->t_submit;
+    return j->status;
+}
+
+int gl_server_drain(gl_server *srv) {
+    if (!srv) return set_error(GL_E_ARG, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    srv->cv_done.wait(lk, [&] { return srv->queue.empty() && srv->inflight == 0; });
+    return GL_OK;
+}
+
+int gl_server_stats_get(gl_server *srv, gl_server_stats *out) {
+    if (!srv || !out) return set_error(GL_E_ARG, "null pointer");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    gl_server_stats st{};
+    st.submitted = srv->submitted;
+    st.completed = srv->completed;
+    st.failed = srv->failed;
+    st.rejected = srv->rejected;
+    std::vector<double> l = srv->latencies;
+    std::sort(l.begin(), l.end());
+    if (!l.empty()) {
+        auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
+        st.p50_ms = pct(0.50);
+        st.p99_ms = pct(0.99);
+        double sum = 0;
+        for (double v : l) sum += v;
+        st.mean_ms = sum / l.size();
+        st.max_ms = l.back();
+    }
+    st.elements = srv->elements;
+    st.busy_seconds = srv->busy_ms / 1e3;
+    st.peak_bytes = srv->peak;
+    *out = st;
+    return GL_OK;
+}
+
+void gl_server_destroy(gl_server *srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        srv->cv_work.notify_all();
+    }
+    if (srv->worker.joinable()) srv->worker.join();
+    for (Slot &sl : srv->slots)
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    delete srv;
+}
+
+}  // extern "C"

@@ -130,6 +130,62 @@ int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint
 int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup,
                   uint64_t *d_lde, uint8_t *d_layers, size_t cap, size_t *out_len, void *stream);
 
+/* ---- Captured batch plans (hipGraph) -------------------------------------
+ * A mixed batch over fixed device buffers, planned once: descriptors are
+ * uploaded at creation and the launch sequence is captured as a hipGraph, so
+ * each gl_batch_plan_launch is one graph launch with no host work.  Results
+ * equal gl_ntt_mixed_dev on the same buffers. */
+typedef struct gl_batch_plan gl_batch_plan;
+int gl_batch_plan_create(uint64_t *const *d, const size_t *n, size_t k, int inverse,
+                         gl_batch_plan **out);
+int gl_batch_plan_launch(gl_batch_plan *plan, void *stream);
+void gl_batch_plan_destroy(gl_batch_plan *plan);
+
+/* ---- Serving: continuous batching of NTT / commit jobs ------------------
+ * A server owns `slots` HIP streams and an HBM budget.  Submitted jobs wait
+ * in FIFO order and are admitted as soon as a slot is free and their device
+ * footprint fits in the unused budget, so work that arrives while other jobs
+ * run overlaps with them on the device.  Twiddle tables and plans stay
+ * resident across jobs.  Job buffers are host memory unless `device` is set.
+ * Results are the same as running each job alone. */
+#define GL_JOB_NTT 0
+#define GL_JOB_INTT 1
+#define GL_JOB_COMMIT 2
+
+typedef struct {
+    int kind;             /* GL_JOB_NTT / GL_JOB_INTT / GL_JOB_COMMIT */
+    int device;           /* nonzero: all buffers below are device pointers */
+    uint64_t *data;       /* NTT: count*n words, transformed in place; COMMIT: rows x cols input */
+    size_t n;             /* NTT length, or commit rows */
+    size_t count;         /* NTT: transforms back to back; COMMIT: columns */
+    unsigned blowup;      /* COMMIT only */
+    uint64_t *lde_out;    /* COMMIT only: (blowup*rows) x cols words */
+    uint8_t *layers_out;  /* COMMIT only: gl_merkle_tree_size(blowup*rows*cols*8) bytes */
+} gl_job;
+
+typedef struct {
+    uint64_t submitted, completed, failed, rejected;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion latency */
+    double elements;                        /* words transformed (NTT) / LDE words (commit) */
+    double busy_seconds;                    /* wall time with at least one job in flight */
+    size_t peak_bytes;                      /* peak admitted device footprint */
+} gl_server_stats;
+
+typedef struct gl_server gl_server;
+/* hbm_budget_bytes = 0: no budget limit. slots in [1, 16]. */
+gl_server *gl_server_create(size_t hbm_budget_bytes, int slots);
+/* Queue a job; GL_E_NOMEM if its footprint alone exceeds the budget, GL_E_SIZE /
+ * GL_E_ARG for invalid jobs.  The job's buffers must stay valid until done. */
+int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket);
+/* Nonblocking: *done = 1 when the job finished; returns the job's status. */
+int gl_server_poll(gl_server *srv, uint64_t ticket, int *done);
+/* Block until the job finished; latency_ms (may be NULL) = submit -> done. */
+int gl_server_wait(gl_server *srv, uint64_t ticket, double *latency_ms);
+/* Block until every submitted job finished. */
+int gl_server_drain(gl_server *srv);
+int gl_server_stats_get(gl_server *srv, gl_server_stats *out);
+void gl_server_destroy(gl_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
