@@ -49,6 +49,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing; run `python -m goldilocks_amd.build`")
+        # torch ships its own libamdhip64 (same SONAME).  Loading torch first
+        # lets this library bind to that copy, so the process has one HIP
+        # runtime; loading ours first would make torch load a second one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         u64, sz, i32, vp = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
         psz = ctypes.POINTER(sz)
