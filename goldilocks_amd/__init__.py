@@ -74,6 +74,8 @@ def lib():
             "gl_fill_random_dev": (i32, [vp, sz, u64, vp]),
             "gl_clear_cache": (None, []),
             "gl_plan_passes": (i32, [u64, vp, vp, vp, sz, psz]),
+            "gl_variants": (sz, [vp, vp, vp, sz]),
+            "gl_plan_set": (i32, [u64, vp, sz]),
             "gl_k12": (i32, [vp, sz, vp, sz]),
             "gl_next_layer_size": (sz, [sz]),
             "gl_merkle_tree_size": (sz, [sz]),
@@ -189,6 +191,21 @@ def plan_info(n):
     _check(lib().gl_plan_passes(n, r1, r2, t, cap, ctypes.byref(npass)))
     return [{"R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
              "kernel": f"ntt_pass_kernel<{r1[i]}, {r2[i]}, 256, {t[i]}>"} for i in range(npass.value)]
+
+
+def variants():
+    """Compiled pass kernels: list of {"id", "R1", "R2", "R", "T"}."""
+    n = lib().gl_variants(None, None, None, 0)
+    r1, r2, t = (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)()
+    lib().gl_variants(r1, r2, t, n)
+    return [{"id": i, "R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i]} for i in range(n)]
+
+
+def plan_set(n, variant_ids):
+    """Install an explicit schedule for length n (empty list: default)."""
+    k = len(variant_ids)
+    ids = (ctypes.c_uint32 * max(k, 1))(*variant_ids)
+    _check(lib().gl_plan_set(n, ids, k))
 
 
 def ntt_cols(matrix, inverse=False):

@@ -1,0 +1,132 @@
+"""Autotuner for NTT schedules.
+
+For a length n it enumerates pass lists built from the compiled kernel
+variants (radix splits and instances-per-workgroup), times each on the
+device with HIP events, checks that every candidate reproduces the default
+schedule's output bit for bit, installs the fastest with gl_plan_set and
+records it in a JSON cache (GOLDILOCKS_TUNE_CACHE or
+~/.cache/goldilocks_amd/tune.json) that load_cache() re-installs.
+
+Usage: python -m goldilocks_amd.autotune 20 24 27   (log2 lengths)
+"""
+import itertools
+import json
+import os
+import sys
+
+import goldilocks_amd as gl
+
+CACHE = os.environ.get("GOLDILOCKS_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "goldilocks_amd", "tune.json"))
+
+
+def candidate_plans(n, max_extra_passes=1, min_radix=16):
+    vs = [v for v in gl.variants() if v["R"] >= min_radix or v["R"] == n]
+    by_r = {}
+    for v in vs:
+        by_r.setdefault(v["R"], []).append(v["id"])
+    radices = sorted(r for r in by_r if n % r == 0)
+    out = []
+    best_p = None
+    for p in range(1, 5):
+        found = []
+        for combo in itertools.product(radices, repeat=p):
+            prod = 1
+            for r in combo:
+                prod *= r
+            if prod != n:
+                continue
+            for ids in itertools.product(*[by_r[r] for r in combo]):
+                found.append(list(ids))
+        if found and best_p is None:
+            best_p = p
+        if best_p is not None:
+            out.extend(found)
+            if p >= best_p + max_extra_passes:
+                break
+    return out
+
+
+def time_plan(torch, x, n, ids, reps=20):
+    if ids is not None:
+        gl.plan_set(n, ids)
+    y = x.clone()
+    gl.ntt(y)  # warm (and plan install check)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        gl.ntt(y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    z = x.clone()
+    gl.ntt(z)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, z
+
+
+def tune(n, verbose=True, max_candidates=400):
+    import torch
+    x = torch.empty(n, dtype=torch.int64, device="cuda")
+    gl.fill_random(x, 12345)
+    gl.plan_set(n, [])
+    base_ms, ref = time_plan(torch, x, n, None)
+    default = [p["kernel"] for p in gl.plan_info(n)]
+    results = [(base_ms, None)]
+    cands = candidate_plans(n)[:max_candidates]
+    for ids in cands:
+        ms, z = time_plan(torch, x, n, ids)
+        if not torch.equal(z, ref):
+            raise RuntimeError(f"schedule {ids} for n={n} is not bit-exact")
+        results.append((ms, ids))
+    gl.plan_set(n, [])
+    results.sort(key=lambda r: r[0])
+    best_ms, best = results[0]
+    if best is not None:
+        gl.plan_set(n, best)
+    if verbose:
+        vs = {v["id"]: v for v in gl.variants()}
+        def desc(ids):
+            return "default " + str(default) if ids is None else \
+                " + ".join(f"{vs[i]['R1']}x{vs[i]['R2']}/T{vs[i]['T']}" for i in ids)
+        print(f"n={n}: {len(cands)} candidates; default {base_ms * 1e3:.1f} us", flush=True)
+        for ms, ids in results[:8]:
+            print(f"   {ms * 1e3:9.1f} us  {desc(ids)}", flush=True)
+    return best, best_ms, base_ms
+
+
+def save(entries):
+    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+    data = {}
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            data = json.load(f)
+    data.update({str(k): v for k, v in entries.items()})
+    with open(CACHE, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def load_cache(path=CACHE):
+    """Install every tuned schedule from the cache file; returns how many."""
+    if not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        data = json.load(f)
+    k = 0
+    for n, ids in data.items():
+        if ids:
+            gl.plan_set(int(n), ids)
+            k += 1
+    return k
+
+
+if __name__ == "__main__":
+    import torch  # noqa: F401
+    gl.lib().gl_init(0)
+    found = {}
+    for arg in sys.argv[1:]:
+        n = 1 << int(arg)
+        best, best_ms, base_ms = tune(n)
+        found[n] = best
+    save(found)

@@ -152,6 +152,24 @@ int gl_fill_random_dev(uint64_t *d, size_t n, uint64_t seed, void *stream) {
 
 void gl_clear_cache(void) { clear_caches(); }
 
+size_t gl_variants(uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap) {
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    size_t total = (size_t)cnt + 1;
+    for (size_t i = 0; i < total && i < cap; i++) {
+        const PassVariant *v = i < (size_t)cnt ? &vs[i] : rader257_variant();
+        if (r1) r1[i] = (uint32_t)v->R1;
+        if (r2) r2[i] = (uint32_t)v->R2;
+        if (t) t[i] = (uint32_t)v->T;
+    }
+    return total;
+}
+
+int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants) {
+    if (nvariants && !variant_ids) return set_error(GL_E_ARG, "null pointer");
+    return set_plan(n, variant_ids, nvariants);
+}
+
 int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses) {
     const Plan *plan = nullptr;
     int rc = get_plan(n, &plan);

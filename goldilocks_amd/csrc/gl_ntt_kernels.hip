@@ -125,10 +125,167 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8).  Remap so that consecutive logical blocks, which touch
+// neighbouring words of the same cache lines, run on one XCD and share its L2.
+__device__ __forceinline__ u32 xcd_block(u32 b, u32 grid) {
+    const u32 per = grid / 8;
+    if (b >= per * 8) return b;
+    return (b % 8) * per + b / 8;
+}
+
+// Three-stage pass: R = R1*R2*R3, DFT_R by decimation in frequency with
+// register codelets DFT_R1, DFT_R2, DFT_R3 and two LDS exchanges.
+//   stage 1, task (t, r2, r3): DFT_R1 over r1, times w_R^(k1*(r2*R3 + r3))
+//   stage 2, task (t, k1, r3): DFT_R2 over r2, times w_(R2*R3)^(k2*r3)
+//   stage 3, task (t, k1, k2): DFT_R3 over r3 -> k = k1 + R1*k2 + R1*R2*k3
+// Lanes run over consecutive instances first for loads; for stores the
+// lane order follows the output layout (k1 first when S == 1, where each
+// instance's outputs are contiguous).
+template <int R3>
+__device__ __forceinline__ int sw(int c, int key) {
+    if constexpr (is_pow2(R3)) return c ^ (key & (R3 - 1));
+    else return c;
+}
+
+template <int R1, int R2, int R3, int NT, int T>
+__device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds) {
+    constexpr int R23 = R2 * R3, R = R1 * R23;
+    constexpr int TS = R + 1;
+    static_assert(T * R1 * R3 == NT, "stage 2 runs one task per thread");
+    constexpr int TA = T * R23, QA = (TA + NT - 1) / NT;
+    constexpr int TC = T * R1 * R2, QC = (TC + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const u32 I0 = blk * (u32)T;
+    // ---- stage 1
+#pragma unroll 1
+    for (int q = 0; q < QA; q++) {
+        const int ta = tid + q * NT;
+        if (ta >= TA) break;
+        const int t = ta % T, r3 = (ta / T) % R3, r2 = ta / (T * R3);
+        const u32 I = I0 + (u32)t;
+        u64 x[R1];
+        if (I < a.total_inst) {
+            const u32 b = I / a.ninst;
+            const u32 i = I - b * a.ninst;
+            const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+            const u64 *src = a.in + (u64)b * a.bstride;
+            if (a.flags & FLAG_REVERSE) {
+#pragma unroll
+                for (int r1 = 0; r1 < R1; r1++) {
+                    u64 pos = (u64)m + (u64)a.Mp * (u64)(r1 * R23 + r2 * R3 + r3);
+                    x[r1] = src[s + a.S * (pos ? a.n_rev - pos : 0)];
+                }
+            } else {
+                const u64 *p = src + (u64)i + a.SMp * (u64)(r2 * R3 + r3);
+#pragma unroll
+                for (int r1 = 0; r1 < R1; r1++) x[r1] = p[a.SMp * (u64)(r1 * R23)];
+            }
+        } else {
+#pragma unroll
+            for (int r1 = 0; r1 < R1; r1++) x[r1] = 0;
+        }
+        dft<R1>(x);
+        const int c = r2 * R3 + r3;
+#pragma unroll
+        for (int k1 = 1; k1 < R1; k1++) x[k1] = mul(x[k1], a.tw_inner[(k1 * c) % R]);
+        u64 *row = lds + t * TS + r2 * R3 + sw<R3>(r3, r2);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) row[k1 * R23] = x[k1];
+    }
+    __syncthreads();
+    // ---- stage 2 (reads complete before any thread overwrites)
+    {
+        const int t = tid % T, r3 = (tid / T) % R3, k1 = tid / (T * R3);
+        u64 y[R2];
+        const u64 *row = lds + t * TS + k1 * R23;
+#pragma unroll
+        for (int r2 = 0; r2 < R2; r2++) y[r2] = row[r2 * R3 + sw<R3>(r3, r2)];
+        dft<R2>(y);
+#pragma unroll
+        for (int k2 = 1; k2 < R2; k2++) y[k2] = mul(y[k2], a.tw_inner[(R1 * k2 * r3) % R]);
+        __syncthreads();
+        u64 *wrow = lds + t * TS + k1 * R23 + sw<R3>(r3, k1);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) wrow[k2 * R3] = y[k2];
+    }
+    __syncthreads();
+    // ---- stage 3
+#pragma unroll 1
+    for (int q = 0; q < QC; q++) {
+        const int tc = tid + q * NT;
+        if (tc >= TC) break;
+        int t, k1, k2;
+        if (a.S == 1) {
+            k1 = tc % R1;
+            t = (tc / R1) % T;
+            k2 = tc / (R1 * T);
+        } else {
+            t = tc % T;
+            k1 = (tc / T) % R1;
+            k2 = tc / (T * R1);
+        }
+        const u32 I = I0 + (u32)t;
+        u64 z[R3];
+        const u64 *row = lds + t * TS + k1 * R23 + k2 * R3;
+#pragma unroll
+        for (int r3 = 0; r3 < R3; r3++) z[r3] = row[sw<R3>(r3, k1)];
+        dft<R3>(z);
+        if (I < a.total_inst) {
+            const u32 b = I / a.ninst;
+            const u32 i = I - b * a.ninst;
+            const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+            const u32 kk = (u32)(k1 + R1 * k2);
+            u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)kk + (u64)R * (u64)m);
+#pragma unroll
+            for (int k3 = 0; k3 < R3; k3++) {
+                u64 v = z[k3];
+                if (a.flags & FLAG_TWIDDLE) {
+                    u64 e = (u64)m * (u64)(kk + R1 * R2 * k3);
+                    v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                }
+                if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                dst[a.S * (u64)(R1 * R2 * k3)] = v;
+            }
+        }
+    }
+}
+
+template <int R1, int R2, int R3, int NT, int T>
+__global__ __launch_bounds__(NT) void ntt_pass3_kernel(PassArgs a) {
+    __shared__ u64 lds[T * (R1 * R2 * R3 + 1)];
+    pass3_body<R1, R2, R3, NT, T>(a, xcd_block(blockIdx.x, gridDim.x), lds);
+}
+
+template <int R1, int R2, int R3, int NT, int T>
+__global__ __launch_bounds__(NT) void ntt_pass3_grouped_kernel(const PassArgs *__restrict__ descs,
+                                                               const u32 *__restrict__ starts, u32 nd) {
+    __shared__ u64 lds[T * (R1 * R2 * R3 + 1)];
+    const u32 b = blockIdx.x;
+    u32 lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) / 2;
+        if (starts[mid] <= b) lo = mid; else hi = mid;
+    }
+    const PassArgs a = descs[lo];
+    pass3_body<R1, R2, R3, NT, T>(a, b - starts[lo], lds);
+}
+
+template <int R1, int R2, int R3, int NT, int T>
+static void launch_pass3(const PassArgs &a, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((ntt_pass3_kernel<R1, R2, R3, NT, T>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <int R1, int R2, int R3, int NT, int T>
+static void launch_pass3_grouped(const PassArgs *descs, const u32 *starts, u32 nd, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((ntt_pass3_grouped_kernel<R1, R2, R3, NT, T>), dim3(grid), dim3(NT), 0, s, descs, starts,
+                       nd);
+}
+
 template <int R1, int R2, int NT, int T>
 __global__ __launch_bounds__(NT) void ntt_pass_kernel(PassArgs a) {
     __shared__ u64 lds[lds_words<R1, R2, T>()];
-    pass_body<R1, R2, NT, T>(a, blockIdx.x, lds);
+    pass_body<R1, R2, NT, T>(a, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 // Grouped launch: descriptor j owns blocks [starts[j], starts[j+1]).  Used
@@ -173,6 +330,14 @@ constexpr int pick_T(int R1, int R2, int NT) {
      &launch_pass<R1, R2, 256, pick_T(R1, R2, 256)>,                                     \
      &launch_grouped<R1, R2, 256, pick_T(R1, R2, 256)>}
 
+// Explicit (NT, T) variants for tuning; R2 holds R2*R3 for three-stage ones.
+#define GL_VARIANT_T(R1, R2, NT, T)                                                     \
+    {R1, R2, T, NT, (size_t)(T) * ((R1) * (R2) + 1) * 8, &launch_pass<R1, R2, NT, T>,   \
+     &launch_grouped<R1, R2, NT, T>}
+#define GL_VARIANT3(R1, R2, R3, NT, T)                                                   \
+    {R1, (R2) * (R3), T, NT, (size_t)(T) * ((R1) * (R2) * (R3) + 1) * 8,                 \
+     &launch_pass3<R1, R2, R3, NT, T>, &launch_pass3_grouped<R1, R2, R3, NT, T>}
+
 static const PassVariant kVariants[] = {
     // powers of two
     GL_VARIANT(2, 1), GL_VARIANT(4, 1), GL_VARIANT(8, 1), GL_VARIANT(16, 1),
@@ -188,6 +353,10 @@ static const PassVariant kVariants[] = {
     GL_VARIANT(8, 3), GL_VARIANT(8, 5), GL_VARIANT(8, 15), GL_VARIANT(8, 17),
     GL_VARIANT(4, 3), GL_VARIANT(4, 5), GL_VARIANT(4, 15), GL_VARIANT(4, 17),
     GL_VARIANT(2, 3), GL_VARIANT(2, 5), GL_VARIANT(2, 15), GL_VARIANT(2, 17),
+    // tuning alternatives for the large power-of-two passes
+    GL_VARIANT_T(64, 64, 128, 2), GL_VARIANT_T(64, 64, 64, 1), GL_VARIANT_T(32, 32, 128, 4),
+    GL_VARIANT_T(16, 16, 512, 32), GL_VARIANT3(16, 16, 16, 1024, 4), GL_VARIANT3(16, 16, 16, 512, 2),
+    GL_VARIANT3(8, 8, 8, 512, 8), GL_VARIANT3(16, 16, 8, 512, 4), GL_VARIANT3(16, 8, 8, 512, 4),
 };
 
 const PassVariant *pass_variants(int *count) {

@@ -387,6 +387,32 @@ int get_plan(u64 n, const Plan **out) {
     return GL_OK;
 }
 
+int set_plan(u64 n, const u32 *ids, size_t nids) {
+    if (n == 0 || (P - 1) % n != 0) return set_error(GL_E_SIZE, "plan_set: size does not divide p - 1");
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (nids == 0) {
+        g_plans.erase(n);
+        return GL_OK;
+    }
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    std::unique_ptr<Plan> p(new Plan());
+    p->n = n;
+    u64 prod = 1;
+    for (size_t i = 0; i < nids; i++) {
+        const PassVariant *v = nullptr;
+        if (ids[i] < (u32)cnt) v = &vs[ids[i]];
+        else if (ids[i] == (u32)cnt) v = rader257_variant();
+        if (!v) return set_error(GL_E_ARG, "plan_set: unknown variant id");
+        u64 R = (u64)v->R1 * (u64)v->R2;
+        p->passes.push_back({v, R});
+        prod *= R;
+    }
+    if (prod != n) return set_error(GL_E_ARG, "plan_set: radices do not multiply to n");
+    g_plans[n] = std::move(p);
+    return GL_OK;
+}
+
 size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride) {
     return plan.passes.size() > 1 ? (size_t)(count * bstride) : 0;
 }

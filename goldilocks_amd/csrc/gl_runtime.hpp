@@ -71,6 +71,10 @@ struct Plan {
 };
 // Returns GL_OK and a pass list for length n, GL_E_SIZE or GL_E_UNSUPPORTED.
 int get_plan(u64 n, const Plan **out);
+// Install an explicit pass list (variant indices as listed by gl_variants;
+// index == variant count means the radix-257 Rader pass); empty restores
+// the planner's default.
+int set_plan(u64 n, const u32 *ids, size_t nids);
 
 // Pass p of a plan as launch arguments.  `src_is_data` tracks the ping-pong
 // state: pass 0 reads data; non-final passes alternate data <-> tmp; the

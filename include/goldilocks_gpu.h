@@ -94,6 +94,15 @@ int gl_fill_random_dev(uint64_t *d, size_t n, uint64_t seed, void *stream);
  * reference's counterpart is the plan tree strategy(n) builds.) */
 int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses);
 
+/* Compiled pass kernel variants: entry i runs DFT_(r1*r2) per instance with
+ * t instances per workgroup.  Returns the count; fills up to cap entries. */
+size_t gl_variants(uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap);
+
+/* Replace the schedule for length n by the given variant indices (product
+ * of their radices must be n); used by the autotuner.  nvariants = 0 restores
+ * the planner's default. */
+int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants);
+
 /* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46). */
 void gl_clear_cache(void);
 
