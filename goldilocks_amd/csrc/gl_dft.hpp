@@ -116,8 +116,8 @@ constexpr u64 rader_tw(int q) {
 template <int Pr, int Q>
 GL_HD void rader_scale(u64 *c) {
     if constexpr (Q < Pr - 1) {
-        constexpr u64 t = rader_tw<Pr>(Q);
-        c[Q] = mul(c[Q], t);
+        constexpr u64 t = cx_mul(rader_tw<Pr>(Q), 0xffffffffull);  // Montgomery form
+        c[Q] = mont_mul(c[Q], t);
         rader_scale<Pr, Q + 1>(c);
     }
 }

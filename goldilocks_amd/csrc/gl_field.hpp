@@ -68,6 +68,22 @@ GL_HD u64 mul(u64 a, u64 b) {
     return reduce159(lo, (u32)hi, hi >> 32);
 }
 
+// Montgomery product a * b * 2^-64 mod p (generic.rs:52-54,114-124):
+// canonical whenever a * b < p * 2^64 (e.g. a < p).  Twiddle tables are kept
+// in Montgomery form (w * 2^64 mod p), so mont_mul(x, table[i]) = x * w.
+GL_HD u64 mont_mul(u64 a, u64 b) {
+    u64 x0, x1;
+    mul128(a, b, x0, x1);
+    u64 s = x0 + (x0 << 32);
+    u64 e = s < x0;
+    u64 t = s - (s >> 32) - e;
+    u64 r = x1 - t;
+    return (x1 < t) ? r - EPS : r;
+}
+
+// Montgomery form of a canonical value: a * 2^64 mod p.
+GL_HD u64 to_mont(u64 a) { return mul(a, EPS); }
+
 // a * 2^S for canonical a and a compile-time S (generic.rs:171-193).
 template <int S_>
 GL_HD u64 shl(u64 a) {

@@ -41,7 +41,10 @@ enum : u32 {
     FLAG_TWIDDLE = 1u,  // apply w_M^(m*k) to the outputs (Mp > 1)
     FLAG_REVERSE = 2u,  // read x[(n - j) mod n] (first pass of intt)
     FLAG_SCALE = 4u,    // multiply outputs by `scale` (1/n for intt)
+    FLAG_TWIDDLE_FULL = 8u,  // tw_lo is a full table: output k of instance m uses tw_lo[k + R*m]
 };
+// All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
+// are applied with mont_mul.
 
 struct PassVariant {
     int R1, R2, T, NT;
@@ -59,7 +62,7 @@ const PassVariant *rader257_variant();
 
 // Elementwise kernels (gl_ntt_kernels.hip).
 void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
-void launch_scale_reverse(u64 *v, u64 n, u64 count, u64 scale, hipStream_t s);
+void launch_pow_table(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s);
 void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 

@@ -70,7 +70,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         dft<R1>(x);
         if constexpr (R2 > 1) {
 #pragma unroll
-            for (int ka = 1; ka < R1; ka++) x[ka] = mul(x[ka], a.tw_inner[(ka * rb) % R]);
+            for (int ka = 1; ka < R1; ka++) x[ka] = mont_mul(x[ka], a.tw_inner[(ka * rb) % R]);
             u64 *row = lds + t * TS;
 #pragma unroll
             for (int ka = 0; ka < R1; ka++) row[ka * R2 + lds_col<R2>(rb, ka)] = x[ka];
@@ -80,11 +80,13 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
             for (int k = 0; k < R1; k++) {
                 u64 v = x[k];
-                if (a.flags & FLAG_TWIDDLE) {
+                if (a.flags & FLAG_TWIDDLE_FULL) {
+                    v = mont_mul(v, a.tw_lo[(u64)k + (u64)R * (u64)m]);
+                } else if (a.flags & FLAG_TWIDDLE) {
                     u64 e = (u64)m * (u64)k;
-                    v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                    v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
                 }
-                if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
                 dst[a.S * (u64)k] = v;
             }
         }
@@ -113,11 +115,13 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
                 for (int kb = 0; kb < R2; kb++) {
                     u64 v = z[kb];
-                    if (a.flags & FLAG_TWIDDLE) {
+                    if (a.flags & FLAG_TWIDDLE_FULL) {
+                        v = mont_mul(v, a.tw_lo[(u64)(ka + R1 * kb) + (u64)R * (u64)m]);
+                    } else if (a.flags & FLAG_TWIDDLE) {
                         u64 e = (u64)m * (u64)(ka + R1 * kb);
-                        v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                        v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
                     }
-                    if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                    if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
                     dst[a.S * (u64)(R1 * kb)] = v;
                 }
             }
@@ -188,7 +192,7 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
         dft<R1>(x);
         const int c = r2 * R3 + r3;
 #pragma unroll
-        for (int k1 = 1; k1 < R1; k1++) x[k1] = mul(x[k1], a.tw_inner[(k1 * c) % R]);
+        for (int k1 = 1; k1 < R1; k1++) x[k1] = mont_mul(x[k1], a.tw_inner[(k1 * c) % R]);
         u64 *row = lds + t * TS + r2 * R3 + sw<R3>(r3, r2);
 #pragma unroll
         for (int k1 = 0; k1 < R1; k1++) row[k1 * R23] = x[k1];
@@ -203,7 +207,7 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
         for (int r2 = 0; r2 < R2; r2++) y[r2] = row[r2 * R3 + sw<R3>(r3, r2)];
         dft<R2>(y);
 #pragma unroll
-        for (int k2 = 1; k2 < R2; k2++) y[k2] = mul(y[k2], a.tw_inner[(R1 * k2 * r3) % R]);
+        for (int k2 = 1; k2 < R2; k2++) y[k2] = mont_mul(y[k2], a.tw_inner[(R1 * k2 * r3) % R]);
         __syncthreads();
         u64 *wrow = lds + t * TS + k1 * R23 + sw<R3>(r3, k1);
 #pragma unroll
@@ -240,11 +244,13 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
 #pragma unroll
             for (int k3 = 0; k3 < R3; k3++) {
                 u64 v = z[k3];
-                if (a.flags & FLAG_TWIDDLE) {
+                if (a.flags & FLAG_TWIDDLE_FULL) {
+                    v = mont_mul(v, a.tw_lo[(u64)(kk + R1 * R2 * k3) + (u64)R * (u64)m]);
+                } else if (a.flags & FLAG_TWIDDLE) {
                     u64 e = (u64)m * (u64)(kk + R1 * R2 * k3);
-                    v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+                    v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
                 }
-                if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+                if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
                 dst[a.S * (u64)(R1 * R2 * k3)] = v;
             }
         }
@@ -389,27 +395,17 @@ void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s) {
     hipLaunchKernelGGL(fill_random_kernel, dim3(grid), dim3(256), 0, s, v, n, seed);
 }
 
-// intt pre-step for items handled without a fused first pass:
-// v = scale * v[(n - j) mod n] per item, in place (pairs swapped by one thread).
-__global__ void scale_reverse_kernel(u64 *v, u64 n, u64 count, u64 scale) {
-    u64 half = n / 2 + 1;  // j in [0, half) pairs with n - j
-    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < half * count;
-         g += (u64)gridDim.x * blockDim.x) {
-        u64 b = g / half, j = g % half;
-        u64 *x = v + b * n;
-        u64 k = j ? n - j : 0;
-        if (k < j) continue;
-        u64 a = x[j], c = x[k];
-        x[j] = mul(c, scale);
-        if (k != j) x[k] = mul(a, scale);
-    }
+// Full power table in Montgomery form: out[e] = lo[e & mask] * hi[e >> lo_bits]
+// for e < count (lo, hi in Montgomery form).
+__global__ void pow_table_kernel(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits) {
+    for (u64 e = blockIdx.x * (u64)blockDim.x + threadIdx.x; e < count; e += (u64)gridDim.x * blockDim.x)
+        out[e] = mont_mul(lo[e & ((1ull << lo_bits) - 1)], hi[e >> lo_bits]);
 }
 
-void launch_scale_reverse(u64 *v, u64 n, u64 count, u64 scale, hipStream_t s) {
-    u64 work = (n / 2 + 1) * count;
-    unsigned grid = (unsigned)((work + 255) / 256 < 65536 ? (work + 255) / 256 : 65536);
+void launch_pow_table(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s) {
+    unsigned grid = (unsigned)((count + 255) / 256 < 65536 ? (count + 255) / 256 : 65536);
     if (grid == 0) return;
-    hipLaunchKernelGGL(scale_reverse_kernel, dim3(grid), dim3(256), 0, s, v, n, count, scale);
+    hipLaunchKernelGGL(pow_table_kernel, dim3(grid), dim3(256), 0, s, out, count, lo, hi, lo_bits);
 }
 
 // Coset extension for the commit: ext[r][c] = coef[r][c] * g^r for r < rows,
@@ -420,7 +416,7 @@ __global__ void coset_extend_kernel(const u64 *coef, u64 *ext, u64 rows, u64 col
     for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
         u64 r = g / cols;
         u64 v = 0;
-        if (r < rows) v = mul(coef[g], mul(lo[r & ((1ull << lo_bits) - 1)], hi[r >> lo_bits]));
+        if (r < rows) v = mont_mul(coef[g], mont_mul(lo[r & ((1ull << lo_bits) - 1)], hi[r >> lo_bits]));
         ext[g] = v;
     }
 }

@@ -49,7 +49,7 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
     if (j == 0 && valid) x0 = load(0);
     dft<16>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) x[k1] = mul(x[k1], w256[(k1 * j) & 255]);
+    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w256[(k1 * j) & 255]);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) rowA[k1 * 16 + (j ^ k1)] = x[k1];
     __syncthreads();
@@ -62,7 +62,7 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
 #pragma unroll
     for (int k2 = 0; k2 < 16; k2++) {
         const int k = j + 16 * k2;
-        u64 c = mul(x[k2], spec[k]);
+        u64 c = mont_mul(x[k2], spec[k]);
         if (k == 0) {
             X0 = add(x0, x[0]);
             c = add(c, x0);
@@ -76,7 +76,7 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
     for (int q1 = 0; q1 < 16; q1++) x[q1] = rowB[rd_col(q1, j)];
     dft<16>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) x[k1] = mul(x[k1], w256[(k1 * j) & 255]);
+    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w256[(k1 * j) & 255]);
     __syncthreads();  // rowA is reused: every sub-step b read has completed
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) rowA[k1 * 16 + (j ^ k1)] = x[k1];
@@ -88,11 +88,13 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
     if (!valid) return;
     u64 *dst = a.out + (u64)b * a.bstride + s + a.S * 257ull * (u64)m;
     auto store = [&](u64 kk, u64 v) {
-        if (a.flags & FLAG_TWIDDLE) {
+        if (a.flags & FLAG_TWIDDLE_FULL) {
+            v = mont_mul(v, a.tw_lo[kk + 257ull * (u64)m]);
+        } else if (a.flags & FLAG_TWIDDLE) {
             u64 e = (u64)m * kk;
-            v = mul(v, mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+            v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
         }
-        if (a.flags & FLAG_SCALE) v = mul(v, a.scale);
+        if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
         dst[a.S * kk] = v;
     };
 #pragma unroll

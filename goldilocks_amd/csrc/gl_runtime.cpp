@@ -3,6 +3,7 @@
 #include "gl_runtime.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -192,12 +193,12 @@ PowTable power_table(u64 base, u64 count) {
     std::vector<u64> lo(L), hi(H ? H : 1);
     u64 w = 1;
     for (u64 e = 0; e < L; e++) {
-        lo[e] = w;
+        lo[e] = to_mont(w);
         w = mul(w, base);
     }
     u64 wl = w, v = 1;  // wl = base^L
     for (u64 j = 0; j < hi.size(); j++) {
-        hi[j] = v;
+        hi[j] = to_mont(v);
         v = mul(v, wl);
     }
     PowTable t;
@@ -210,6 +211,41 @@ PowTable power_table(u64 base, u64 count) {
 
 PowTable root_table(u64 M) { return power_table(root(M), M); }
 
+static std::map<u64, const u64 *> g_full_tables;
+
+const u64 *full_root_table(u64 M) {
+    PowTable t = root_table(M);
+    if (!t.lo || !t.hi) return nullptr;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_full_tables.find(M);
+    if (it != g_full_tables.end()) return it->second;
+    void *d = nullptr;
+    if (hipMalloc(&d, M * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    launch_pow_table((u64 *)d, M, t.lo, t.hi, t.lo_bits, nullptr);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_tw_allocs.push_back(d);
+    g_full_tables[M] = (const u64 *)d;
+    return (const u64 *)d;
+}
+
+// Full twiddle tables cost M words of HBM and one streaming read per output;
+// they replace the two-level product (one multiply per output).  Sizes up to
+// 2^28 words use them unless GOLDILOCKS_TWIDDLE_FULL_MAX says otherwise.
+static u64 full_table_max() {
+    static u64 v = [] {
+        const char *e = getenv("GOLDILOCKS_TWIDDLE_FULL_MAX");
+        return e ? (u64)strtoull(e, nullptr, 10) : (u64)1 << 28;
+    }();
+    return v;
+}
+
 const u64 *inner_table(u64 R) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_inner_tables.find(R);
@@ -217,7 +253,7 @@ const u64 *inner_table(u64 R) {
     std::vector<u64> t(R);
     u64 w = root(R), v = 1;
     for (u64 e = 0; e < R; e++) {
-        t[e] = v;
+        t[e] = to_mont(v);
         v = mul(v, w);
     }
     const u64 *d = upload(t);
@@ -248,6 +284,7 @@ const u64 *rader257_table() {
         for (int q = 0; q < 256; q++) acc = add(acc, mul(c[q], t[(q * k) & 255]));
         t[256 + k] = acc;
     }
+    for (int e = 0; e < 512; e++) t[e] = to_mont(t[e]);
     u64 pk = 1, pi = 1;
     for (int q = 0; q < 256; q++) {
         t[512 + q] = pk;
@@ -274,8 +311,13 @@ static const PassVariant *find_variant(int R1, int R2) {
 // Pass with `bits` power-of-two bits and odd codelet g (g = 1: none).
 static const PassVariant *variant_for(int bits, int g) {
     if (g == 1) {
+        // Power-of-two passes: two register stages up to 2^8, three stages
+        // (16-register codelets, four waves per SIMD) above; chosen from the
+        // autotuner's sweeps on MI355X.
+        static const int pref[13][2] = {{0, 0},  {2, 1},  {2, 2},  {4, 2},   {4, 4},   {8, 4},  {8, 8},
+                                        {16, 8}, {16, 16}, {8, 64}, {16, 64}, {16, 128}, {16, 256}};
         if (bits < 1 || bits > 12) return nullptr;
-        return find_variant(1 << ((bits + 1) / 2), 1 << (bits / 2));
+        return find_variant(pref[bits][0], pref[bits][1]);
     }
     if (bits == 0) return find_variant(g, 1);
     if (bits > 6) return nullptr;
@@ -442,17 +484,23 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         if (!a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     }
     if (Mp > 1) {
-        PowTable t = root_table(M);
-        if (!t.lo || !t.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-        a.tw_lo = t.lo;
-        a.tw_hi = t.hi;
-        a.lo_bits = t.lo_bits;
-        a.flags |= FLAG_TWIDDLE;
+        const u64 *full = M <= full_table_max() ? full_root_table(M) : nullptr;
+        if (full) {
+            a.tw_lo = full;
+            a.flags |= FLAG_TWIDDLE_FULL;
+        } else {
+            PowTable t = root_table(M);
+            if (!t.lo || !t.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+            a.tw_lo = t.lo;
+            a.tw_hi = t.hi;
+            a.lo_bits = t.lo_bits;
+            a.flags |= FLAG_TWIDDLE;
+        }
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
     if (inverse && last) {
         a.flags |= FLAG_SCALE;
-        a.scale = inv(n % P);
+        a.scale = to_mont(inv(n % P));
     }
     grid = (unsigned)((total + pp.v->T - 1) / pp.v->T);
     src = dst;
@@ -491,6 +539,7 @@ void clear_caches() {
         g_tw_allocs.clear();
         g_pow_tables.clear();
         g_inner_tables.clear();
+        g_full_tables.clear();
         g_rader257 = nullptr;
     }
     release_scratch();
