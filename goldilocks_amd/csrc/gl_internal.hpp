@@ -62,7 +62,8 @@ const PassVariant *rader257_variant();
 
 // Elementwise kernels (gl_ntt_kernels.hip).
 void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
-void launch_pow_table(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s);
+void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 *hi, u32 lo_bits,
+                           hipStream_t s);
 void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 

@@ -395,17 +395,19 @@ void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s) {
     hipLaunchKernelGGL(fill_random_kernel, dim3(grid), dim3(256), 0, s, v, n, seed);
 }
 
-// Full power table in Montgomery form: out[e] = lo[e & mask] * hi[e >> lo_bits]
-// for e < count (lo, hi in Montgomery form).
-__global__ void pow_table_kernel(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits) {
-    for (u64 e = blockIdx.x * (u64)blockDim.x + threadIdx.x; e < count; e += (u64)gridDim.x * blockDim.x)
-        out[e] = mont_mul(lo[e & ((1ull << lo_bits) - 1)], hi[e >> lo_bits]);
+// Twiddle matrix in output order, Montgomery form: out[k + R*m] = w^(m*k)
+// for m < count / R, k < R, where w^e = lo[e & mask] * hi[e >> lo_bits].
+__global__ void twiddle_matrix_kernel(u64 *out, u64 count, u64 R, const u64 *lo, const u64 *hi, u32 lo_bits) {
+    for (u64 idx = blockIdx.x * (u64)blockDim.x + threadIdx.x; idx < count; idx += (u64)gridDim.x * blockDim.x) {
+        const u64 m = idx / R, k = idx - m * R, e = m * k;
+        out[idx] = mont_mul(lo[e & ((1ull << lo_bits) - 1)], hi[e >> lo_bits]);
+    }
 }
 
-void launch_pow_table(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s) {
+void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s) {
     unsigned grid = (unsigned)((count + 255) / 256 < 65536 ? (count + 255) / 256 : 65536);
     if (grid == 0) return;
-    hipLaunchKernelGGL(pow_table_kernel, dim3(grid), dim3(256), 0, s, out, count, lo, hi, lo_bits);
+    hipLaunchKernelGGL(twiddle_matrix_kernel, dim3(grid), dim3(256), 0, s, out, count, R, lo, hi, lo_bits);
 }
 
 // Coset extension for the commit: ext[r][c] = coef[r][c] * g^r for r < rows,

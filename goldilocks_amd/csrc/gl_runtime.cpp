@@ -211,27 +211,30 @@ PowTable power_table(u64 base, u64 count) {
 
 PowTable root_table(u64 M) { return power_table(root(M), M); }
 
-static std::map<u64, const u64 *> g_full_tables;
+static std::map<std::pair<u64, u64>, const u64 *> g_full_tables;
 
-const u64 *full_root_table(u64 M) {
+// Output-order twiddles of a pass with radix R on sub-problems of length M:
+// entry k + R*m = w_M^(m*k) (Montgomery form), M words.
+const u64 *twiddle_matrix(u64 M, u64 R) {
     PowTable t = root_table(M);
     if (!t.lo || !t.hi) return nullptr;
     std::lock_guard<std::mutex> lk(g_tw_mu);
-    auto it = g_full_tables.find(M);
+    auto key = std::make_pair(M, R);
+    auto it = g_full_tables.find(key);
     if (it != g_full_tables.end()) return it->second;
     void *d = nullptr;
     if (hipMalloc(&d, M * 8) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
-    launch_pow_table((u64 *)d, M, t.lo, t.hi, t.lo_bits, nullptr);
+    launch_twiddle_matrix((u64 *)d, M, R, t.lo, t.hi, t.lo_bits, nullptr);
     if (hipDeviceSynchronize() != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(d);
         return nullptr;
     }
     g_tw_allocs.push_back(d);
-    g_full_tables[M] = (const u64 *)d;
+    g_full_tables[key] = (const u64 *)d;
     return (const u64 *)d;
 }
 
@@ -484,7 +487,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         if (!a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     }
     if (Mp > 1) {
-        const u64 *full = M <= full_table_max() ? full_root_table(M) : nullptr;
+        const u64 *full = M <= full_table_max() ? twiddle_matrix(M, R) : nullptr;
         if (full) {
             a.tw_lo = full;
             a.flags |= FLAG_TWIDDLE_FULL;
