@@ -17,6 +17,40 @@
 
 namespace gl {
 
+// Store the NK outputs v[j] (output index k = k0 + j*kstep) of instance m,
+// applying the pass's output twiddle and scale.  The twiddle mode is tested
+// once, outside the unrolled loops, so each loop issues all its table loads
+// before the multiplies.
+template <int NK>
+__device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 m, u32 k0, u32 kstep,
+                                              u32 R, u64 *v) {
+    if (a.flags & FLAG_TWIDDLE_FULL) {
+        const u64 *tw = a.tw_lo + (u64)R * (u64)m + k0;
+        u64 w[NK];
+#pragma unroll
+        for (int j = 0; j < NK; j++) w[j] = tw[(u64)j * kstep];
+#pragma unroll
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], w[j]);
+    } else if (a.flags & FLAG_TWIDDLE) {
+        const u64 mask = (1ull << a.lo_bits) - 1;
+        u64 wl[NK], wh[NK];
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+            const u64 e = (u64)m * (u64)(k0 + j * kstep);
+            wl[j] = a.tw_lo[e & mask];
+            wh[j] = a.tw_hi[e >> a.lo_bits];
+        }
+#pragma unroll
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], mont_mul(wl[j], wh[j]));
+    }
+    if (a.flags & FLAG_SCALE) {
+#pragma unroll
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], a.scale);
+    }
+#pragma unroll
+    for (int j = 0; j < NK; j++) dst[dstep * (u64)j] = v[j];
+}
+
 template <int R2>
 __device__ __forceinline__ int lds_col(int rb, int ka) {
     if constexpr (is_pow2(R2)) return rb ^ (ka & (R2 - 1));
@@ -77,18 +111,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         } else if (valid) {
             // single sub-step pass: k = ka
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * (u64)R * (u64)m;
-#pragma unroll
-            for (int k = 0; k < R1; k++) {
-                u64 v = x[k];
-                if (a.flags & FLAG_TWIDDLE_FULL) {
-                    v = mont_mul(v, a.tw_lo[(u64)k + (u64)R * (u64)m]);
-                } else if (a.flags & FLAG_TWIDDLE) {
-                    u64 e = (u64)m * (u64)k;
-                    v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
-                }
-                if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
-                dst[a.S * (u64)k] = v;
-            }
+            store_outputs<R1>(a, dst, a.S, m, 0, 1, R, x);
         }
     }
     if constexpr (R2 > 1) {
@@ -99,7 +122,16 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         for (int q = 0; q < QB; q++) {
             const int tb = tid + q * NT;
             if (tb >= TB) break;
-            const int t = tb % T, ka = tb / T;
+            // S == 1: an instance's outputs are contiguous, so lanes run over
+            // ka first; otherwise over consecutive instances.
+            int t, ka;
+            if (a.S == 1) {
+                ka = tb % R1;
+                t = tb / R1;
+            } else {
+                t = tb % T;
+                ka = tb / T;
+            }
             const u32 I = I0 + (u32)t;
             u64 z[R2];
             const u64 *row = lds + t * TS + ka * R2;
@@ -112,18 +144,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
                 const u32 s = (u32)(i % (u32)a.S);
                 const u32 m = (u32)(i / (u32)a.S);
                 u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
-#pragma unroll
-                for (int kb = 0; kb < R2; kb++) {
-                    u64 v = z[kb];
-                    if (a.flags & FLAG_TWIDDLE_FULL) {
-                        v = mont_mul(v, a.tw_lo[(u64)(ka + R1 * kb) + (u64)R * (u64)m]);
-                    } else if (a.flags & FLAG_TWIDDLE) {
-                        u64 e = (u64)m * (u64)(ka + R1 * kb);
-                        v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
-                    }
-                    if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
-                    dst[a.S * (u64)(R1 * kb)] = v;
-                }
+                store_outputs<R2>(a, dst, a.S * (u64)R1, m, (u32)ka, (u32)R1, (u32)R, z);
             }
         }
     }
@@ -241,18 +262,7 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
             const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
             const u32 kk = (u32)(k1 + R1 * k2);
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)kk + (u64)R * (u64)m);
-#pragma unroll
-            for (int k3 = 0; k3 < R3; k3++) {
-                u64 v = z[k3];
-                if (a.flags & FLAG_TWIDDLE_FULL) {
-                    v = mont_mul(v, a.tw_lo[(u64)(kk + R1 * R2 * k3) + (u64)R * (u64)m]);
-                } else if (a.flags & FLAG_TWIDDLE) {
-                    u64 e = (u64)m * (u64)(kk + R1 * R2 * k3);
-                    v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
-                }
-                if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
-                dst[a.S * (u64)(R1 * R2 * k3)] = v;
-            }
+            store_outputs<R3>(a, dst, a.S * (u64)(R1 * R2), m, kk, (u32)(R1 * R2), (u32)R, z);
         }
     }
 }

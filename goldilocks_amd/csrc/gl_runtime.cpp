@@ -238,13 +238,14 @@ const u64 *twiddle_matrix(u64 M, u64 R) {
     return (const u64 *)d;
 }
 
-// Full twiddle tables cost M words of HBM and one streaming read per output;
-// they replace the two-level product (one multiply per output).  Sizes up to
-// 2^28 words use them unless GOLDILOCKS_TWIDDLE_FULL_MAX says otherwise.
+// A full twiddle matrix costs M words and one read per output; it replaces
+// the two-level product (one multiply per output).  It pays off while the
+// matrix stays cache resident (M up to 2^20 words = 8 MiB by default, set by
+// GOLDILOCKS_TWIDDLE_FULL_MAX); larger passes use the two-level tables.
 static u64 full_table_max() {
     static u64 v = [] {
         const char *e = getenv("GOLDILOCKS_TWIDDLE_FULL_MAX");
-        return e ? (u64)strtoull(e, nullptr, 10) : (u64)1 << 28;
+        return e ? (u64)strtoull(e, nullptr, 10) : (u64)1 << 20;
     }();
     return v;
 }
