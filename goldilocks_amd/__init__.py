@@ -76,6 +76,19 @@ def lib():
             "gl_plan_passes": (i32, [u64, vp, vp, vp, sz, psz]),
             "gl_variants": (sz, [vp, vp, vp, sz]),
             "gl_plan_set": (i32, [u64, vp, sz]),
+            "gl_batch_plan_create": (i32, [vp, vp, sz, i32, vp]),
+            "gl_batch_plan_launch": (i32, [vp, vp]),
+            "gl_batch_plan_destroy": (None, [vp]),
+            "gl_server_create": (vp, [sz, i32]),
+            "gl_server_submit": (i32, [vp, vp, ctypes.POINTER(u64)]),
+            "gl_server_poll": (i32, [vp, u64, ctypes.POINTER(i32)]),
+            "gl_server_wait": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
+            "gl_server_drain": (i32, [vp]),
+            "gl_server_stats_get": (i32, [vp, vp]),
+            "gl_server_destroy": (None, [vp]),
+            "gl_pow_solve": (i32, [vp, u64, u64, ctypes.POINTER(u64)]),
+            "gl_convolve": (i32, [vp, vp, sz]),
+            "gl_convolve_dev": (i32, [vp, vp, sz, vp]),
             "gl_k12": (i32, [vp, sz, vp, sz]),
             "gl_next_layer_size": (sz, [sz]),
             "gl_merkle_tree_size": (sz, [sz]),
@@ -325,3 +338,164 @@ def commit(matrix, blowup=2):
     _check(lib().gl_commit(m.ctypes.data_as(ctypes.c_void_p), rows, cols, blowup,
                            lde.ctypes.data_as(ctypes.c_void_p), out, need, ctypes.byref(got)))
     return lde, _split_layers(out.raw[:need], ext * cols * 8)
+
+
+# ---------------------------------------------------------------- PoW / convolution
+
+def threshold_from_bits(bits):
+    """pcs/src/pow.rs:15-17."""
+    return (2**64 - 1) - int(2.0 ** (64.0 - bits))
+
+
+def pow_verify(challenge: bytes, threshold: int, nonce: int) -> bool:
+    """pcs/src/pow.rs:32-37 (one K12 hash on the GPU)."""
+    h = k12(bytes(challenge) + int(nonce).to_bytes(8, "little"), 8)
+    return int.from_bytes(h, "little") >= threshold
+
+
+def pow_solve(challenge: bytes, threshold: int, max_nonce=1 << 40) -> int:
+    """pcs/src/pow.rs:39-60: the smallest satisfying nonce (GPU scan)."""
+    out = ctypes.c_uint64(0)
+    _check(lib().gl_pow_solve(bytes(challenge), threshold, max_nonce, ctypes.byref(out)))
+    return out.value
+
+
+def convolve(a, b_ntt):
+    """ntt/src/convolve.rs:13-20: a <- intt(ntt(a) * b_ntt), Field (Montgomery)
+    pointwise product on raw words.  numpy arrays or GPU tensors, in place."""
+    n = _len(a)
+    if _len(b_ntt) != n:
+        raise ValueError("lengths differ")
+    if _is_torch(a):
+        _check(lib().gl_convolve_dev(_dev_ptr(a), _dev_ptr(b_ntt), n, _stream_ptr(a)))
+    else:
+        _check(lib().gl_convolve(_host_u64(a).ctypes.data_as(ctypes.c_void_p),
+                                 _host_u64(np.ascontiguousarray(b_ntt, dtype=np.uint64)).ctypes.data_as(ctypes.c_void_p), n))
+    return a
+
+
+# ---------------------------------------------------------------- batch plans
+
+class BatchPlan:
+    """A mixed-size batch over fixed GPU tensors, planned once and replayed
+    as one captured hipGraph launch (gl_batch_plan_*)."""
+
+    def __init__(self, tensors, inverse=False):
+        k = len(tensors)
+        self._tensors = list(tensors)  # keep the buffers alive
+        ns = (ctypes.c_size_t * k)(*[_len(t) for t in tensors])
+        ptrs = (ctypes.c_void_p * k)(*[_dev_ptr(t).value for t in tensors])
+        h = ctypes.c_void_p()
+        _check(lib().gl_batch_plan_create(ptrs, ns, k, int(inverse), ctypes.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(lib().gl_batch_plan_launch(self._h, ctypes.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if self._h:
+            lib().gl_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- serving
+
+GL_JOB_NTT, GL_JOB_INTT, GL_JOB_COMMIT = 0, 1, 2
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("device", ctypes.c_int), ("data", ctypes.c_void_p),
+                ("n", ctypes.c_size_t), ("count", ctypes.c_size_t), ("blowup", ctypes.c_uint),
+                ("lde_out", ctypes.c_void_p), ("layers_out", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("elements", ctypes.c_double),
+                ("busy_seconds", ctypes.c_double), ("peak_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Server:
+    """Continuous-batching NTT / commit server (gl_server_*).  Jobs are
+    admitted as streams and HBM budget free up; their work overlaps on the
+    device.  Buffers (numpy for host jobs, GPU tensors for device jobs) must
+    stay alive until the job is done."""
+
+    def __init__(self, hbm_budget_bytes=0, slots=4):
+        h = lib().gl_server_create(hbm_budget_bytes, slots)
+        if not h:
+            raise GoldilocksError(GL_E_DEVICE, lib().gl_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+        self._keep = {}
+
+    def _submit(self, job, keep):
+        t = ctypes.c_uint64(0)
+        _check(lib().gl_server_submit(self._h, ctypes.byref(job), ctypes.byref(t)))
+        self._keep[t.value] = keep
+        return t.value
+
+    def submit_ntt(self, values, n=None, inverse=False):
+        """Transform `values` (numpy uint64 or GPU tensor) in place as
+        len(values) // n transforms of length n."""
+        n = n or _len(values)
+        dev = _is_torch(values)
+        ptr = _dev_ptr(values).value if dev else _host_u64(values).ctypes.data
+        job = _Job(GL_JOB_INTT if inverse else GL_JOB_NTT, int(dev), ptr, n, _len(values) // n, 0, None, None)
+        return self._submit(job, values)
+
+    def submit_commit(self, matrix, blowup, lde_out, layers_out):
+        """Commit a rows x cols matrix; lde_out ((blowup*rows) x cols) and
+        layers_out (merkle_tree_size bytes) receive the results."""
+        rows, cols = matrix.shape
+        dev = _is_torch(matrix)
+        if dev:
+            ptrs = (_dev_ptr(matrix).value, _dev_ptr(lde_out).value, ctypes.c_void_p(layers_out.data_ptr()).value)
+        else:
+            ptrs = (_host_u64(matrix).ctypes.data, _host_u64(lde_out).ctypes.data, layers_out.ctypes.data)
+        job = _Job(GL_JOB_COMMIT, int(dev), ptrs[0], rows, cols, blowup, ptrs[1], ptrs[2])
+        return self._submit(job, (matrix, lde_out, layers_out))
+
+    def poll(self, ticket):
+        done = ctypes.c_int(0)
+        _check(lib().gl_server_poll(self._h, ticket, ctypes.byref(done)))
+        return bool(done.value)
+
+    def wait(self, ticket):
+        """Block until the job is done; returns its submit-to-done latency in ms."""
+        lat = ctypes.c_double(0)
+        rc = lib().gl_server_wait(self._h, ticket, ctypes.byref(lat))
+        self._keep.pop(ticket, None)
+        _check(rc)
+        return lat.value
+
+    def drain(self):
+        _check(lib().gl_server_drain(self._h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().gl_server_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().gl_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -260,6 +260,61 @@ int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
     return GL_OK;
 }
 
+int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_nonce, uint64_t *nonce) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!challenge32 || !nonce) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf d(256);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    unsigned long long *best = (unsigned long long *)d.p;
+    uint8_t *dch = (uint8_t *)d.p + 64;
+    const unsigned long long none = ~0ull;
+    GL_HIP(hipMemcpyAsync(best, &none, 8, hipMemcpyHostToDevice, s));
+    GL_HIP(hipMemcpyAsync(dch, challenge32, 32, hipMemcpyHostToDevice, s));
+    unsigned long long found = none;
+    const u64 chunk = 1ull << 26;
+    for (u64 start = 0; start < max_nonce; start += chunk) {
+        u64 cnt = max_nonce - start < chunk ? max_nonce - start : chunk;
+        launch_pow_search(dch, threshold, start, cnt, best, s);
+        GL_HIP(hipGetLastError());
+        GL_HIP(hipMemcpyAsync(&found, best, 8, hipMemcpyDeviceToHost, s));
+        GL_HIP(hipStreamSynchronize(s));
+        if (found != none) break;  // every smaller nonce was tested in earlier chunks
+    }
+    if (found == none) return set_error(GL_E_ARG, "pow: no nonce below max_nonce");
+    *nonce = found;
+    return GL_OK;
+}
+
+int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n && (!d_a || !d_b_ntt)) return set_error(GL_E_ARG, "null device pointer");
+    hipStream_t s = as_stream(stream);
+    if ((rc = ntt_dev_impl(d_a, n, 1, 1, false, s))) return rc;
+    launch_pointwise_mont(d_a, d_b_ntt, n, s);
+    GL_HIP(hipGetLastError());
+    return ntt_dev_impl(d_a, n, 1, 1, true, s);
+}
+
+int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!gl_is_divisor(n)) return set_error(GL_E_SIZE, "convolve: length does not divide p - 1");
+    if (!a || !b_ntt) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf d(2 * n * 8);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    u64 *da = d.as<u64>(), *db = da + n;
+    GL_HIP(hipMemcpyAsync(da, a, n * 8, hipMemcpyHostToDevice, s));
+    GL_HIP(hipMemcpyAsync(db, b_ntt, n * 8, hipMemcpyHostToDevice, s));
+    if ((rc = gl_convolve_dev(da, db, n, s))) return rc;
+    GL_HIP(hipMemcpyAsync(a, da, n * 8, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
 // ---------------------------------------------------------------- commit
 
 int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde,

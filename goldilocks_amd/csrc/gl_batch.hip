@@ -27,6 +27,7 @@ struct Group {
 
 struct Schedule {
     std::vector<const Plan *> plans;
+    std::vector<size_t> composite;
     std::vector<u64> tmp_off;
     u64 tmp_words = 0;
     std::vector<Group> groups;
@@ -44,6 +45,11 @@ int plan_batch(uint64_t *const *d, const size_t *n, size_t k, bool inverse, Sche
     for (size_t i = 0; i < k; i++) {
         if (inverse && n[i] <= 1) continue;
         if ((rc = get_plan(n[i], &sc.plans[i]))) return rc;
+        if (plan_is_composite(*sc.plans[i])) {
+            sc.composite.push_back(i);  // not groupable: run on its own
+            sc.plans[i] = nullptr;
+            continue;
+        }
         if (!d[i] && !sc.plans[i]->passes.empty()) return set_error(GL_E_ARG, "null device pointer");
         if (sc.plans[i]->passes.size() > 1) {
             sc.tmp_off[i] = sc.tmp_words;
@@ -134,6 +140,8 @@ extern "C" int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, i
     hipStream_t s = (hipStream_t)stream;
     Schedule sc;
     if ((rc = plan_batch(d, n, k, inverse != 0, sc))) return rc;
+    for (size_t i : sc.composite)
+        if ((rc = gl_ntt_dev(d[i], n[i], 1, inverse, stream))) return rc;
     u64 *tmp = stream_scratch(s, sc.tmp_words, 0);
     if (sc.tmp_words && !tmp) return set_error(GL_E_NOMEM, "scratch allocation failed");
     if ((rc = build_groups(d, n, k, inverse != 0, tmp, sc))) return rc;
@@ -206,6 +214,10 @@ extern "C" int gl_batch_plan_create(uint64_t *const *d, const size_t *n, size_t 
     if ((rc = plan_batch(d, n, k, inverse != 0, sc))) {
         gl_batch_plan_destroy(p);
         return rc;
+    }
+    if (!sc.composite.empty()) {
+        gl_batch_plan_destroy(p);
+        return set_error(GL_E_UNSUPPORTED, "batch plans cannot capture lengths with a 65537 factor");
     }
     if (sc.tmp_words && hipMalloc(&p->tmp, sc.tmp_words * 8) != hipSuccess) {
         (void)hipGetLastError();

@@ -64,6 +64,10 @@ const PassVariant *rader257_variant();
 void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
 void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 *hi, u32 lo_bits,
                            hipStream_t s);
+void launch_rader_big_gather(const PassArgs &a, const u32 *permk, u64 *C, u64 *x0, hipStream_t s);
+void launch_rader_big_pointwise(u64 *C, u64 *x0, const u64 *spec, u64 ninst, hipStream_t s);
+void launch_rader_big_scatter(const PassArgs &a, const u32 *permi, const u64 *D, const u64 *X0, hipStream_t s);
+void launch_pointwise_mont(u64 *a, const u64 *b, u64 n, hipStream_t s);
 void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 

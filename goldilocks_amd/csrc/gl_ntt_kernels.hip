@@ -420,6 +420,103 @@ void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 
     hipLaunchKernelGGL(twiddle_matrix_kernel, dim3(grid), dim3(256), 0, s, out, count, R, lo, hi, lo_bits);
 }
 
+// ---- radix-65537 Rader pass pieces (rader.rs:65-93 with (gi, gk) = (3, 21846)).
+// Instance I (of the pass) owns C[I*65536 .. +65536) and x0[I].
+
+// C[I][q] = x[gk^q mod 65537], x0[I] = x[0] for the pass's input view.
+__global__ void rader_big_gather_kernel(PassArgs a, const u32 *__restrict__ permk, u64 *__restrict__ C,
+                                        u64 *__restrict__ x0) {
+    const u64 total = (u64)a.total_inst * 65536ull;
+    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        const u32 I = (u32)(g >> 16), q = (u32)(g & 65535);
+        const u32 b = I / a.ninst, i = I - b * a.ninst;
+        const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+        const u64 *src = a.in + (u64)b * a.bstride;
+        auto at = [&](u64 r) {
+            u64 pos = (u64)m + (u64)a.Mp * r;
+            if (a.flags & FLAG_REVERSE) pos = pos ? a.n_rev - pos : 0;
+            return src[s + a.S * pos];
+        };
+        C[g] = at(permk[q]);
+        if (q == 0) x0[I] = at(0);
+    }
+}
+
+// X0 = x0 + A[0]; C = A * TW (Montgomery spectrum); C[0] += x0.  X0 is kept
+// in x0[] for the scatter.
+__global__ void rader_big_pointwise_kernel(u64 *__restrict__ C, u64 *__restrict__ x0, const u64 *__restrict__ spec,
+                                           u64 ninst) {
+    const u64 total = ninst * 65536ull;
+    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        const u32 q = (u32)(g & 65535);
+        u64 c = mont_mul(C[g], spec[q]);
+        if (q == 0) {
+            const u64 I = g >> 16, v = x0[I];
+            x0[I] = add(v, C[g]);
+            c = add(c, v);
+        }
+        C[g] = c;
+    }
+}
+
+// out[s + S*(k + 65537*m)] = D[q] (k = gi^q mod 65537), out at k = 0 = X0,
+// with the pass's output twiddle and scale.
+__global__ void rader_big_scatter_kernel(PassArgs a, const u32 *__restrict__ permi, const u64 *__restrict__ D,
+                                         const u64 *__restrict__ X0) {
+    const u64 total = (u64)a.total_inst * 65537ull;
+    for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        const u32 I = (u32)(g / 65537ull), j = (u32)(g - (u64)I * 65537ull);
+        const u32 b = I / a.ninst, i = I - b * a.ninst;
+        const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+        u64 v, k;
+        if (j == 65536) {
+            v = X0[I];
+            k = 0;
+        } else {
+            v = D[(u64)I * 65536ull + j];
+            k = permi[j];
+        }
+        if (a.flags & FLAG_TWIDDLE_FULL) {
+            v = mont_mul(v, a.tw_lo[k + 65537ull * (u64)m]);
+        } else if (a.flags & FLAG_TWIDDLE) {
+            const u64 e = (u64)m * k;
+            v = mont_mul(v, mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]));
+        }
+        if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
+        a.out[(u64)b * a.bstride + s + a.S * (k + 65537ull * (u64)m)] = v;
+    }
+}
+
+static unsigned grid_for(u64 work) {
+    u64 g = (work + 255) / 256;
+    return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
+}
+
+void launch_rader_big_gather(const PassArgs &a, const u32 *permk, u64 *C, u64 *x0, hipStream_t s) {
+    hipLaunchKernelGGL(rader_big_gather_kernel, dim3(grid_for((u64)a.total_inst * 65536)), dim3(256), 0, s, a, permk,
+                       C, x0);
+}
+void launch_rader_big_pointwise(u64 *C, u64 *x0, const u64 *spec, u64 ninst, hipStream_t s) {
+    hipLaunchKernelGGL(rader_big_pointwise_kernel, dim3(grid_for(ninst * 65536)), dim3(256), 0, s, C, x0, spec,
+                       ninst);
+}
+void launch_rader_big_scatter(const PassArgs &a, const u32 *permi, const u64 *D, const u64 *X0, hipStream_t s) {
+    hipLaunchKernelGGL(rader_big_scatter_kernel, dim3(grid_for((u64)a.total_inst * 65537)), dim3(256), 0, s, a,
+                       permi, D, X0);
+}
+
+// a[i] = a[i] * b[i] as the reference's Field product (Montgomery).
+__global__ void pointwise_mont_kernel(u64 *a, const u64 *b, u64 n) {
+    for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+        a[i] = mont_mul(a[i], b[i]);
+}
+
+void launch_pointwise_mont(u64 *a, const u64 *b, u64 n, hipStream_t s) {
+    unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(pointwise_mont_kernel, dim3(grid), dim3(256), 0, s, a, b, n);
+}
+
 // Coset extension for the commit: ext[r][c] = coef[r][c] * g^r for r < rows,
 // zero for rows <= r < blowup*rows (g^r = lo[r & mask] * hi[r >> lo_bits]).
 __global__ void coset_extend_kernel(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,

@@ -300,6 +300,95 @@ const u64 *rader257_table() {
     return g_rader257;
 }
 
+// Radix-65537 Rader pass: no single kernel (launch == nullptr); run_plan
+// composes gather, two batched 65536-point transforms, a pointwise product
+// with the spectrum and a scatter.
+static const PassVariant kRader65537 = {65537, 1, 1, 256, 0, nullptr, nullptr};
+
+struct BigRaderTables {
+    const u32 *permk = nullptr, *permi = nullptr;
+    const u64 *spec = nullptr;
+};
+static BigRaderTables g_big_rader;
+
+static int big_rader_tables(BigRaderTables &out) {
+    {
+        std::lock_guard<std::mutex> lk(g_tw_mu);
+        if (g_big_rader.spec) {
+            out = g_big_rader;
+            return GL_OK;
+        }
+    }
+    const u64 P2 = 65537, M2 = 65536;
+    std::vector<u32> pk(M2), pi(M2);
+    std::vector<u64> c(M2);
+    u64 w = root(P2), sc = inv(M2);
+    u64 a = 1, b = 1;
+    for (u64 q = 0; q < M2; q++) {
+        pk[q] = (u32)a;
+        pi[q] = (u32)b;
+        c[q] = to_mont(mul(pow(w, b), sc));
+        a = a * 21846 % P2;
+        b = b * 3 % P2;
+    }
+    void *dpk = nullptr, *dpi = nullptr, *dspec = nullptr, *dtmp = nullptr;
+    if (hipMalloc(&dpk, M2 * 4) != hipSuccess || hipMalloc(&dpi, M2 * 4) != hipSuccess ||
+        hipMalloc(&dspec, M2 * 8) != hipSuccess || hipMalloc(&dtmp, M2 * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GL_E_NOMEM, "radix-65537 table allocation failed");
+    }
+    GL_HIP(hipMemcpy(dpk, pk.data(), M2 * 4, hipMemcpyHostToDevice));
+    GL_HIP(hipMemcpy(dpi, pi.data(), M2 * 4, hipMemcpyHostToDevice));
+    GL_HIP(hipMemcpy(dspec, c.data(), M2 * 8, hipMemcpyHostToDevice));
+    // spectrum = DFT_65536(c) (linear, so Montgomery form is preserved)
+    const Plan *p65536 = nullptr;
+    int rc = get_plan(M2, &p65536);
+    if (rc) return rc;
+    if ((rc = run_plan(*p65536, (u64 *)dspec, (u64 *)dtmp, 1, 1, M2, false, nullptr))) return rc;
+    GL_HIP(hipDeviceSynchronize());
+    (void)hipFree(dtmp);
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (!g_big_rader.spec) {
+        g_big_rader.permk = (const u32 *)dpk;
+        g_big_rader.permi = (const u32 *)dpi;
+        g_big_rader.spec = (const u64 *)dspec;
+        g_tw_allocs.push_back(dpk);
+        g_tw_allocs.push_back(dpi);
+        g_tw_allocs.push_back(dspec);
+    }
+    out = g_big_rader;
+    return GL_OK;
+}
+
+static int run_rader_big(const PassArgs &a, hipStream_t stream) {
+    BigRaderTables t;
+    int rc = big_rader_tables(t);
+    if (rc) return rc;
+    const Plan *p65536 = nullptr;
+    if ((rc = get_plan(65536, &p65536))) return rc;
+    const u64 ni = a.total_inst, words = ni * 65536;
+    u64 *C = stream_scratch(stream, words, 2);
+    u64 *Ct = stream_scratch(stream, words, 3);
+    u64 *x0 = stream_scratch(stream, ni, 4);
+    if (!C || !Ct || !x0) return set_error(GL_E_NOMEM, "radix-65537 scratch allocation failed");
+    launch_rader_big_gather(a, t.permk, C, x0, stream);
+    GL_HIP(hipGetLastError());
+    if ((rc = run_plan(*p65536, C, Ct, 1, ni, 65536, false, stream))) return rc;
+    launch_rader_big_pointwise(C, x0, t.spec, ni, stream);
+    GL_HIP(hipGetLastError());
+    // reverse C[1..] and transform again (unscaled: the spectrum holds 1/65536)
+    if ((rc = run_plan(*p65536, C, Ct, 1, ni, 65536, true, stream, false))) return rc;
+    launch_rader_big_scatter(a, t.permi, C, x0, stream);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+bool plan_is_composite(const Plan &plan) {
+    for (const PassPlan &p : plan.passes)
+        if (!p.v->launch) return true;
+    return false;
+}
+
 // ---------------------------------------------------------------- planner
 static std::mutex g_plan_mu;
 static std::map<u64, std::unique_ptr<Plan>> g_plans;
@@ -336,13 +425,16 @@ static bool build_plan(u64 n, Plan &plan) {
     u64 f = n >> k;
     const bool rader257 = f % 257 == 0;
     if (rader257) f /= 257;
+    const bool rader65537 = f % 65537 == 0;
+    if (rader65537) f /= 65537;
     std::vector<int> primes;
     for (u64 q : {3ull, 5ull, 17ull})
         if (f % q == 0) {
             primes.push_back((int)q);
             f /= q;
         }
-    if (f != 1) return false;  // a 65537 factor: not scheduled by this planner
+    if (f != 1) return false;
+    if (rader65537) plan.passes.push_back({&kRader65537, 65537});
     if (rader257) plan.passes.push_back({rader257_variant(), 257});
     if (k == 0 && primes.empty()) return true;
     // Candidate groupings of the odd primes into single-codelet groups.
@@ -502,7 +594,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         }
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
-    if (inverse && last) {
+    if (inverse && last && scale_out) {
         a.flags |= FLAG_SCALE;
         a.scale = to_mont(inv(n % P));
     }
@@ -515,9 +607,10 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
 }
 
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream) {
+             hipStream_t stream, bool scale_out) {
     if (plan.passes.empty() || count == 0) return GL_OK;
     PassCursor c{&plan, data, tmp, S0, count, bstride, inverse};
+    c.scale_out = scale_out;
     c.begin();
     while (!c.done()) {
         const PassVariant *v = plan.passes[c.p].v;
@@ -525,6 +618,10 @@ int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstri
         unsigned grid = 0;
         int rc = c.next(a, grid);
         if (rc) return rc;
+        if (!v->launch) {
+            if ((rc = run_rader_big(a, stream))) return rc;
+            continue;
+        }
         v->launch(a, grid, stream);
         GL_HIP(hipGetLastError());
     }
@@ -545,6 +642,7 @@ void clear_caches() {
         g_inner_tables.clear();
         g_full_tables.clear();
         g_rader257 = nullptr;
+        g_big_rader = BigRaderTables();
     }
     release_scratch();
     pool_trim();

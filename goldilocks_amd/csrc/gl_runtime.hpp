@@ -84,6 +84,7 @@ struct PassCursor {
     u64 *data, *tmp;
     u64 S0, count, bstride;
     bool inverse;
+    bool scale_out = true;  // inverse: multiply the last pass by 1/n
     // evolving state
     size_t p = 0;
     u64 S = 0, M = 0;
@@ -98,7 +99,10 @@ struct PassCursor {
 // of length n (S0 = 1: contiguous transforms; S0 = cols: matrix columns).
 // tmp must hold count*bstride words when the plan has more than one pass.
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream);
+             hipStream_t stream, bool scale_out = true);
+// True when the plan contains a pass that is not a single kernel (the
+// radix-65537 Rader pass), which grouped batch launches cannot include.
+bool plan_is_composite(const Plan &plan);
 size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
 
 void clear_caches();

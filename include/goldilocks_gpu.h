@@ -129,6 +129,18 @@ int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
 int gl_merkle_tree_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
                        size_t *out_len, void *stream);
 
+/* Proof of work (pcs/src/pow.rs:32-60): the smallest nonce with
+ * u64_le(K12(challenge || nonce_le)[0..8]) >= threshold, searched on the GPU
+ * in increasing order up to max_nonce (exclusive).  GL_E_ARG if none. */
+int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_nonce, uint64_t *nonce);
+
+/* ---- Convolution (ntt/src/convolve.rs:13-20) -----------------------------
+ * circular(a, b): a <- intt(ntt(a) * b) with b already in NTT form; the
+ * pointwise product is the reference's Field product (Montgomery, R = 2^64)
+ * on the raw words.  Length n must divide p - 1. */
+int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n);
+int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stream);
+
 /* ---- Commit (no single reference function; composed from ntt::intt,
  * coefficient scaling by GENERATOR^i, ntt::ntt and merkle_tree over the raw
  * bytes as bench/src/main.rs:179-183 hashes them) ---------------------------

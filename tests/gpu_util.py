@@ -11,12 +11,12 @@ def note(msg):
 
 
 def supported_by_planner(n):
-    """Lengths the device planner schedules (odd part from 3, 5, 17, 257)."""
+    """Lengths the device planner schedules (every divisor of p - 1)."""
     if n <= 0:
         return False
     while n % 2 == 0:
         n //= 2
-    for q in (3, 5, 17, 257):
+    for q in (3, 5, 17, 257, 65537):
         if n % q == 0:
             n //= q
     return n == 1

@@ -40,7 +40,7 @@ def test_every_small_size_forward_and_inverse(gl, oracle):
 
 MEDIUM = [2**13, 2**15, 2**16, 2**18, 3 * 2**14, 5 * 2**13, 15 * 2**12, 17 * 2**12, 51 * 2**10,
           85 * 2**9, 255 * 2**8, 17 * 2**16, 3 * 2**17, 257 * 2**10, 257 * 3 * 2**8,
-          257 * 17 * 2**4, 257 * 2**14]
+          257 * 17 * 2**4, 257 * 2**14, 65537, 65537 * 4, 65537 * 3 * 8]
 
 
 @pytest.mark.parametrize("n", MEDIUM)
@@ -67,7 +67,7 @@ def test_2_16_forward_inverse_config(gl, oracle):
 
 
 @pytest.mark.parametrize("n,count", [(64, 37), (1024, 9), (4096, 5), (15 * 64, 11), (2**16, 3), (17, 100),
-                                     (257, 33), (257 * 64, 5)])
+                                     (257, 33), (257 * 64, 5), (65537, 3)])
 def test_batch(gl, oracle, n, count):
     note(f"batch n={n} count={count}")
     x = oracle.fill_random(n * count, 7)
