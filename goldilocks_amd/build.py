@@ -56,9 +56,22 @@ def build(jobs=None, verbose=False):
     return LIB
 
 
+def build_cpp_tests():
+    """Compile tests/cpp/api_test (the C++ mirror's tests) against the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "api_test")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src,
+           "-L", HERE, "-lgoldilocks_amd", "-Wl,-rpath,$ORIGIN/../../goldilocks_amd", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{r.stderr[-4000:]}")
+    return out
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     args = ap.parse_args()
     build(args.jobs, verbose=True)
+    build_cpp_tests()
     sys.exit(0)
