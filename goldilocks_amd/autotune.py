@@ -96,6 +96,29 @@ def tune(n, verbose=True, max_candidates=400):
     return best, best_ms, base_ms
 
 
+TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.txt")
+
+
+def write_tune_file(entries, path=TUNE_FILE):
+    """Write schedules in the text format the library loads at init:
+    "n R1xR2/T R1xR2/T ..." per line (merged with existing lines)."""
+    vs = {v["id"]: v for v in gl.variants()}
+    lines = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if line.strip() and not line.startswith("#"):
+                lines[int(line.split()[0])] = line.strip()
+    for n, ids in entries.items():
+        if ids:
+            lines[int(n)] = f"{n} " + " ".join(f"{vs[i]['R1']}x{vs[i]['R2']}/{vs[i]['T']}" for i in ids)
+        else:
+            lines.pop(int(n), None)
+    with open(path, "w") as f:
+        f.write("# autotuned NTT schedules for gfx950 (python -m goldilocks_amd.autotune --write ...)\n")
+        for n in sorted(lines):
+            f.write(lines[n] + "\n")
+
+
 def save(entries):
     os.makedirs(os.path.dirname(CACHE), exist_ok=True)
     data = {}
@@ -123,10 +146,20 @@ def load_cache(path=CACHE):
 
 if __name__ == "__main__":
     import torch  # noqa: F401
+    args = sys.argv[1:]
+    write = "--write" in args
+    out = None
+    if "--out" in args:
+        out = args[args.index("--out") + 1]
+        args = [a for a in args if a not in ("--out", out)]
+    args = [a for a in args if a != "--write"]
     gl.lib().gl_init(0)
     found = {}
-    for arg in sys.argv[1:]:
-        n = 1 << int(arg)
+    for arg in args:
+        n = int(arg[1:]) if arg.startswith("=") else 1 << int(arg)
         best, best_ms, base_ms = tune(n)
-        found[n] = best
+        # keep the default when the win is within timing noise
+        found[n] = best if best is not None and best_ms < 0.97 * base_ms else None
     save(found)
+    if write or out:
+        write_tune_file(found, out or TUNE_FILE)

@@ -3,7 +3,9 @@
 #include "gl_runtime.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
+#include <dlfcn.h>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -31,6 +33,8 @@ static std::once_flag g_init_once;
 static int g_init_rc = GL_OK;
 static std::string g_init_msg;
 
+static void load_tune_file();
+
 int ensure_init() {
     std::call_once(g_init_once, [] {
         int n = 0;
@@ -51,7 +55,9 @@ int ensure_init() {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
             g_init_rc = GL_E_DEVICE;
             g_init_msg = std::string("library is built for gfx950, device is ") + prop.gcnArchName;
+            return;
         }
+        load_tune_file();
     });
     if (g_init_rc != GL_OK) return set_error(g_init_rc, g_init_msg);
     return GL_OK;
@@ -549,6 +555,55 @@ int set_plan(u64 n, const u32 *ids, size_t nids) {
     if (prod != n) return set_error(GL_E_ARG, "plan_set: radices do not multiply to n");
     g_plans[n] = std::move(p);
     return GL_OK;
+}
+
+// Autotuned schedules: text lines "n R1xR2/T R1xR2/T ..." (variant radices
+// and instances per workgroup, as gl_variants lists them).  Read from
+// GOLDILOCKS_TUNE_FILE, else tune_gfx950.txt next to this library.
+static void load_tune_file() {
+    std::string path;
+    if (const char *e = getenv("GOLDILOCKS_TUNE_FILE")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&load_tune_file, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tune_gfx950.txt";
+        }
+    }
+    FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (!f) return;
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    char line[1024];
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        char *save = nullptr;
+        char *tok = strtok_r(line, " \t\r\n", &save);
+        if (!tok) continue;
+        u64 n = strtoull(tok, nullptr, 10);
+        std::vector<u32> ids;
+        bool ok = true;
+        while ((tok = strtok_r(nullptr, " \t\r\n", &save))) {
+            int r1 = 0, r2 = 0, t = 0;
+            if (sscanf(tok, "%dx%d/%d", &r1, &r2, &t) != 3) {
+                ok = false;
+                break;
+            }
+            int id = -1;
+            for (int i = 0; i < cnt; i++)
+                if (vs[i].R1 == r1 && vs[i].R2 == r2 && vs[i].T == t) id = i;
+            if (r1 == 257 && r2 == 1) id = cnt;
+            if (id < 0) {
+                ok = false;
+                break;
+            }
+            ids.push_back((u32)id);
+        }
+        if (ok && !ids.empty()) (void)set_plan(n, ids.data(), ids.size());
+    }
+    fclose(f);
 }
 
 size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride) {
