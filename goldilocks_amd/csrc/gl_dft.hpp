@@ -89,7 +89,7 @@ GL_HD void dft_pow2(u64 *x) {
 GL_HD void dft3(u64 *x) {
     u64 a = x[0], b = x[1], c = x[2];
     u64 s = add(b, c);
-    u64 u = add(shl<64>(b), shl<128>(c));  // w b + w^2 c
+    u64 u = sub(shl<64>(b), shl<32>(c));  // w b + w^2 c, w^2 = 2^128 = -2^32
     x[0] = add(a, s);
     x[1] = add(a, u);
     x[2] = sub(sub(a, s), u);              // w^2 b + w c = -(b + c) - u

@@ -24,18 +24,47 @@ typedef uint32_t u32;
 constexpr u64 P = 0xffffffff00000001ull;
 constexpr u64 EPS = 0xffffffffull;  // 2^64 mod p = 2^32 - 1
 
-// a + b for canonical a, b (generic.rs:17-26 form: a - (p - b), fix the
-// borrow by adding p, i.e. subtracting 2^32 - 1 modulo 2^64).
-GL_HD u64 add(u64 a, u64 b) {
-    u64 nb = P - b;
-    u64 r = a - nb;
-    return (a < nb) ? r - EPS : r;
+GL_HD u32 lo32(u64 x) { return (u32)x; }
+GL_HD u32 hi32(u64 x) { return (u32)(x >> 32); }
+GL_HD u64 mk64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
+
+// 64-bit a - b and a + b as explicit 32-bit carry chains, so the borrow or
+// carry stays in a lane mask instead of being recomputed by a compare.
+GL_HD u64 sub_borrow(u64 a, u64 b, u32 &c) {
+    u32 c1;
+    u32 r0 = __builtin_subc(lo32(a), lo32(b), 0u, &c1);
+    u32 r1 = __builtin_subc(hi32(a), hi32(b), c1, &c);
+    return mk64(r0, r1);
+}
+GL_HD u64 add_carry(u64 a, u64 b, u32 &c) {
+    u32 c1;
+    u32 r0 = __builtin_addc(lo32(a), lo32(b), 0u, &c1);
+    u32 r1 = __builtin_addc(hi32(a), hi32(b), c1, &c);
+    return mk64(r0, r1);
 }
 
-// a - b for canonical a, b (generic.rs:31-39).
+// r + k * 2^64 reduced to [0, p) for k in {0, 1}, given that the value is
+// below 2^64 + p (then one + (2^32 - 1) step suffices: either the sum
+// carried, or it did not and r >= p, which is exactly when r + EPS carries).
+GL_HD u64 canon_carry(u64 r, u32 k) {
+    u32 k2;
+    u64 t = add_carry(r, EPS, k2);
+    return (k | k2) ? t : r;
+}
+
+// a + b for canonical a, b: same result as generic.rs:17-26.
+GL_HD u64 add(u64 a, u64 b) {
+    u32 k;
+    u64 s = add_carry(a, b, k);
+    return canon_carry(s, k);
+}
+
+// a - b for canonical a, b (generic.rs:31-39): on borrow add p, which is
+// - (2^32 - 1) modulo 2^64.
 GL_HD u64 sub(u64 a, u64 b) {
-    u64 r = a - b;
-    return (a < b) ? r - EPS : r;
+    u32 c;
+    u64 r = sub_borrow(a, b, c);
+    return c ? r + P : r;
 }
 
 GL_HD u64 neg(u64 a) { return a ? P - a : 0; }
@@ -71,35 +100,70 @@ GL_HD u64 mul(u64 a, u64 b) {
 // Montgomery product a * b * 2^-64 mod p (generic.rs:52-54,114-124):
 // canonical whenever a * b < p * 2^64 (e.g. a < p).  Twiddle tables are kept
 // in Montgomery form (w * 2^64 mod p), so mont_mul(x, table[i]) = x * w.
+// With x = x0 + x1 * 2^64: s = x0 + (x0 << 32), t = s - (s >> 32) - carry,
+// result x1 - t, plus p on borrow.
 GL_HD u64 mont_mul(u64 a, u64 b) {
     u64 x0, x1;
     mul128(a, b, x0, x1);
-    u64 s = x0 + (x0 << 32);
-    u64 e = s < x0;
-    u64 t = s - (s >> 32) - e;
-    u64 r = x1 - t;
-    return (x1 < t) ? r - EPS : r;
+    u32 e, c1, c2, bw;
+    u32 s1 = __builtin_addc(hi32(x0), lo32(x0), 0u, &e);
+    u32 t0 = __builtin_subc(lo32(x0), s1, e, &c1);
+    u32 t1 = __builtin_subc(s1, 0u, c1, &c2);
+    u64 r = sub_borrow(x1, mk64(t0, t1), bw);
+    return bw ? r + P : r;
 }
 
 // Montgomery form of a canonical value: a * 2^64 mod p.
 GL_HD u64 to_mont(u64 a) { return mul(a, EPS); }
 
-// a * 2^S for canonical a and a compile-time S (generic.rs:171-193).
+// a * 2^S for any u64 a and a compile-time 0 < S < 96; canonical result.
+// The product a * 2^S = L + H * 2^64 (L = a << S, H = a >> (64 - S)) is
+// reduced with 2^64 = 2^32 - 1 and 2^96 = -1 (generic.rs:144-193); the
+// multiply by 2^32 - 1 folds into one v_mad_u64_u32.
+template <int S>
+GL_HD u64 shl_lt96(u64 a) {
+    static_assert(S > 0 && S < 96, "shift range");
+    if constexpr (S <= 32) {
+        // H < 2^32: value = L + H * EPS, one carry at most.
+        u32 k;
+        u64 r = add_carry((u64)(u32)(a >> (64 - S)) * EPS, a << S, k);
+        return canon_carry(r, k);
+    } else if constexpr (S < 64) {
+        // H = h0 + h1 * 2^32: value = L + h0 * EPS - h1.
+        u64 H = a >> (64 - S);
+        u32 k, b;
+        u64 r = add_carry((u64)lo32(H) * EPS, a << S, k);
+        u64 d = sub_borrow(r, (u64)hi32(H), b);
+        // value = d + (k - b) * 2^64.  k = 1, b = 0 gives d + EPS < p;
+        // k = 0, b = 1 gives d - EPS; k = b needs d reduced (k = b = 1
+        // implies d >= p).
+        u32 plus = (u32)(d >= P) ^ k ^ b, minus = (k ^ 1u) & b;
+        return d + mk64(plus ? 0xffffffffu : minus, minus ? 0xffffffffu : 0u);
+    } else if constexpr (S == 64) {
+        // value = a0 * EPS - a1
+        u32 b;
+        u64 d = sub_borrow((u64)lo32(a) * EPS, (u64)hi32(a), b);
+        return b ? d + P : d;
+    } else {
+        // limbs c2 = bits [64, 96), u = bits [96, 160): value = c2 * EPS - u
+        u32 b;
+        u64 d = sub_borrow((u64)(u32)(a << (S - 64)) * EPS, a >> (96 - S), b);
+        return b ? d + P : d;
+    }
+}
+
+// a * 2^S for canonical a (any integer S; 2^192 = 1, 2^96 = -1).
 template <int S_>
 GL_HD u64 shl(u64 a) {
     constexpr int S = ((S_ % 192) + 192) % 192;
     if constexpr (S == 0) {
         return a;
-    } else if constexpr (S >= 96) {
-        return neg(shl<S - 96>(a));
-    } else if constexpr (S <= 32) {
-        return reduce159(a << S, (u32)(a >> (64 - S)), 0);
-    } else if constexpr (S < 64) {
-        return reduce159(a << S, (u32)(a >> (64 - S)), a >> (96 - S));
-    } else if constexpr (S == 64) {
-        return reduce159(0, (u32)a, a >> 32);
+    } else if constexpr (S == 96) {
+        return neg(a);
+    } else if constexpr (S > 96) {
+        return neg(shl_lt96<S - 96>(a));
     } else {
-        return reduce159(0, (u32)(a << (S - 64)), a >> (96 - S));
+        return shl_lt96<S>(a);
     }
 }
 
