@@ -76,6 +76,9 @@ def lib():
             "gl_plan_passes": (i32, [u64, vp, vp, vp, sz, psz]),
             "gl_variants": (sz, [vp, vp, vp, sz]),
             "gl_plan_set": (i32, [u64, vp, sz]),
+            "gl_plan_variants": (i32, [u64, vp, sz, psz]),
+            "gl_variant_name": (ctypes.c_char_p, [ctypes.c_uint32]),
+            "gl_set_twiddle_full_max": (u64, [u64]),
             "gl_batch_plan_create": (i32, [vp, vp, sz, i32, vp]),
             "gl_batch_plan_launch": (i32, [vp, vp]),
             "gl_batch_plan_destroy": (None, [vp]),
@@ -197,21 +200,36 @@ def clear_cache():
 
 
 def plan_info(n):
-    """Device schedule for length n: [{"R1", "R2", "R", "T", "kernel"}, ...]."""
+    """Device schedule for length n: [{"id", "R1", "R2", "R", "T", "kernel"}, ...]."""
     cap = 8
     r1, r2, t = (ctypes.c_uint32 * cap)(), (ctypes.c_uint32 * cap)(), (ctypes.c_uint32 * cap)()
+    ids = (ctypes.c_uint32 * cap)()
     npass = ctypes.c_size_t(0)
     _check(lib().gl_plan_passes(n, r1, r2, t, cap, ctypes.byref(npass)))
-    return [{"R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
-             "kernel": f"ntt_pass_kernel<{r1[i]}, {r2[i]}, 256, {t[i]}>"} for i in range(npass.value)]
+    _check(lib().gl_plan_variants(n, ids, cap, ctypes.byref(npass)))
+    out = []
+    for i in range(npass.value):
+        name = lib().gl_variant_name(ids[i])
+        out.append({"id": ids[i], "R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
+                    "kernel": name.decode() if name else "?"})
+    return out
 
 
 def variants():
-    """Compiled pass kernels: list of {"id", "R1", "R2", "R", "T"}."""
+    """Compiled pass kernels (and the radix-257 Rader kernel, last):
+    list of {"id", "R1", "R2", "R", "T", "name"}."""
     n = lib().gl_variants(None, None, None, 0)
     r1, r2, t = (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)()
     lib().gl_variants(r1, r2, t, n)
-    return [{"id": i, "R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i]} for i in range(n)]
+    return [{"id": i, "R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
+             "name": lib().gl_variant_name(i).decode()} for i in range(n)]
+
+
+def set_twiddle_full_max(words):
+    """Threshold (sub-problem length) up to which a pass's output twiddles
+    come from one full table; larger passes use the two-level power table.
+    Results are identical; returns the previous value."""
+    return lib().gl_set_twiddle_full_max(int(words))
 
 
 def plan_set(n, variant_ids):

@@ -1,6 +1,10 @@
 // C ABI (include/goldilocks_gpu.h): host-buffer and device-buffer entry
 // points for the NTT, K12 layers, Merkle trees and the commit.
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "gl_field.hpp"
@@ -164,6 +168,57 @@ size_t gl_variants(uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap) {
     }
     return total;
 }
+
+// Variant ids: [0, count) compiled pass kernels, count = radix-257 Rader
+// kernel, count + 1 = radix-65537 Rader composite (several kernels).
+static const PassVariant *variant_by_id(uint32_t id, size_t *count = nullptr) {
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    if (count) *count = (size_t)cnt;
+    if (id < (uint32_t)cnt) return &vs[id];
+    if (id == (uint32_t)cnt) return rader257_variant();
+    if (id == (uint32_t)cnt + 1) return rader65537_variant();
+    return nullptr;
+}
+
+const char *gl_variant_name(uint32_t id) {
+    static std::mutex mu;
+    static std::map<uint32_t, std::string> names;
+    const PassVariant *v = variant_by_id(id);
+    if (!v) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = names.find(id);
+    if (it != names.end()) return it->second.c_str();
+    char buf[96];
+    switch (v->kind) {
+    case VK_THREE_STAGE:
+        snprintf(buf, sizeof buf, "ntt_pass3_kernel<%d, %d, %d, %d, %d>", v->R1, v->R2 / v->R3, v->R3, v->NT, v->T);
+        break;
+    case VK_RADER257: snprintf(buf, sizeof buf, "rader257_kernel"); break;
+    case VK_COMPOSITE: snprintf(buf, sizeof buf, "rader_big_gather/pointwise/scatter + 2 x NTT_65536"); break;
+    default: snprintf(buf, sizeof buf, "ntt_pass_kernel<%d, %d, %d, %d>", v->R1, v->R2, v->NT, v->T);
+    }
+    return names.emplace(id, buf).first->second.c_str();
+}
+
+int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
+    const Plan *plan = nullptr;
+    int rc = get_plan(n, &plan);
+    if (rc) return rc;
+    if (npasses) *npasses = plan->passes.size();
+    size_t cnt = 0;
+    (void)variant_by_id(0, &cnt);
+    for (size_t i = 0; i < plan->passes.size() && i < cap; i++) {
+        const PassVariant *v = plan->passes[i].v;
+        uint32_t id = UINT32_MAX;
+        for (uint32_t j = 0; j < cnt + 2; j++)
+            if (variant_by_id(j) == v) id = j;
+        if (ids) ids[i] = id;
+    }
+    return GL_OK;
+}
+
+uint64_t gl_set_twiddle_full_max(uint64_t words) { return set_full_table_max(words); }
 
 int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants) {
     if (nvariants && !variant_ids) return set_error(GL_E_ARG, "null pointer");

@@ -35,7 +35,37 @@ struct PassArgs {
     const u64 *tw_hi;
     u64 scale;         // multiplier applied to every output when FLAG_SCALE
     u32 flags;
+    // x / ninst and x / S for x < 2^31 as (x * mul) >> sh (fastdiv_params)
+    u32 ninst_mul, ninst_sh, S_mul, S_sh;
 };
+
+// Multiply-shift constants for dividing any x < 2^31 by d >= 1: with
+// l = ceil(log2 d), mul = ceil(2^(31+l) / d) < 2^32 and x * mul / 2^(31+l)
+// exceeds x / d by less than 1/d, so the floor is exact.  d >= 2^32 gives
+// mul = 0 (quotient 0 for every such x).
+inline void fastdiv_params(u64 d, u32 &mul, u32 &sh) {
+    if (d >= (1ull << 32)) {
+        mul = 0;
+        sh = 0;
+        return;
+    }
+    u32 l = 0;
+    while ((1ull << l) < d) l++;
+    unsigned __int128 num = (unsigned __int128)1 << (31 + l);
+    mul = (u32)((num + d - 1) / d);
+    sh = 31 + l;
+}
+
+__host__ __device__ __forceinline__ u32 fastdiv(u32 x, u32 mul, u32 sh) { return (u32)(((u64)x * mul) >> sh); }
+
+// Instance I of a pass -> batch item b, instance i within the item, and its
+// (s, m) with i = s + S*m.
+__device__ __forceinline__ void locate(const PassArgs &a, u32 I, u32 &b, u32 &i, u32 &s, u32 &m) {
+    b = fastdiv(I, a.ninst_mul, a.ninst_sh);
+    i = I - b * a.ninst;
+    m = fastdiv(i, a.S_mul, a.S_sh);
+    s = i - m * (u32)a.S;
+}
 
 enum : u32 {
     FLAG_TWIDDLE = 1u,  // apply w_M^(m*k) to the outputs (Mp > 1)
@@ -46,12 +76,22 @@ enum : u32 {
 // All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
 // are applied with mont_mul.
 
+// Which inner twiddle table a variant reads from PassArgs::tw_inner.
+enum : int {
+    VK_TWO_STAGE = 0,    // R1 x R2 (or one codelet): entry rb*R1 + ka = w_R^(ka*rb)
+    VK_THREE_STAGE = 1,  // R1 x R2 x R3: entry e = w_R^e
+    VK_RADER257 = 2,     // radix-257 Rader tables
+    VK_COMPOSITE = 3,    // radix-65537 Rader: several kernels, no single launch
+};
+
 struct PassVariant {
     int R1, R2, T, NT;
     size_t lds_bytes;
     void (*launch)(const PassArgs &, unsigned grid, hipStream_t);
     // descs/starts in device memory; descriptor j owns blocks [starts[j], starts[j+1])
     void (*launch_grouped)(const PassArgs *descs, const u32 *starts, u32 nd, unsigned grid, hipStream_t);
+    int kind;
+    int R3;  // three-stage variants: last radix (R2 then holds R2 * R3); else 1
 };
 
 // All compiled pass variants (gl_ntt_kernels.hip).

@@ -17,31 +17,36 @@
 
 namespace gl {
 
+// Montgomery form of w_M^e from the two-level power table.
+__device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
+    return mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]);
+}
+
 // Store the NK outputs v[j] (output index k = k0 + j*kstep) of instance m,
-// applying the pass's output twiddle and scale.  The twiddle mode is tested
-// once, outside the unrolled loops, so each loop issues all its table loads
-// before the multiplies.
-template <int NK>
-__device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 m, u32 k0, u32 kstep,
-                                              u32 R, u64 *v) {
+// applying the pass's output twiddle w_M^(m*k) and scale.  With the full
+// table each twiddle is one load at a compile-time offset from the
+// instance's row.  Otherwise the twiddles form the geometric sequence
+// A * B^j with A = w^(m*k0), B = w^(m*kstep): two table lookups per thread,
+// then one product per output to advance (exact field arithmetic, so the
+// result equals the per-output lookup).
+template <int NK, int KSTEP>
+__device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 m, u32 k0, u32 R,
+                                              u64 *v) {
     if (a.flags & FLAG_TWIDDLE_FULL) {
         const u64 *tw = a.tw_lo + (u64)R * (u64)m + k0;
         u64 w[NK];
 #pragma unroll
-        for (int j = 0; j < NK; j++) w[j] = tw[(u64)j * kstep];
+        for (int j = 0; j < NK; j++) w[j] = tw[j * KSTEP];
 #pragma unroll
         for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], w[j]);
     } else if (a.flags & FLAG_TWIDDLE) {
-        const u64 mask = (1ull << a.lo_bits) - 1;
-        u64 wl[NK], wh[NK];
+        u64 w = tw_pow(a, (u64)m * k0);
+        const u64 B = tw_pow(a, (u64)m * KSTEP);
 #pragma unroll
         for (int j = 0; j < NK; j++) {
-            const u64 e = (u64)m * (u64)(k0 + j * kstep);
-            wl[j] = a.tw_lo[e & mask];
-            wh[j] = a.tw_hi[e >> a.lo_bits];
+            v[j] = mont_mul(v[j], w);
+            if (j + 1 < NK) w = mont_mul(w, B);
         }
-#pragma unroll
-        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], mont_mul(wl[j], wh[j]));
     }
     if (a.flags & FLAG_SCALE) {
 #pragma unroll
@@ -51,20 +56,22 @@ __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 d
     for (int j = 0; j < NK; j++) dst[dstep * (u64)j] = v[j];
 }
 
+// LDS tile of the two-sub-step pass: instance t owns TS words, row ka of it
+// RS words (rb along the row).  Odd strides keep both the row-wise writes
+// (lanes over t) and the column-wise reads (lanes over t, or over ka) on
+// distinct banks, and every per-element offset is a compile-time constant.
 template <int R2>
-__device__ __forceinline__ int lds_col(int rb, int ka) {
-    if constexpr (is_pow2(R2)) return rb ^ (ka & (R2 - 1));
-    else return rb;
-}
-
+constexpr int lds_rs() { return R2 | 1; }
+template <int R1, int R2>
+constexpr int lds_ts() { return (R1 * lds_rs<R2>()) | 1; }
 template <int R1, int R2, int T>
-constexpr int lds_words() { return R2 > 1 ? T * (R1 * R2 + 1) : 1; }
+constexpr int lds_words() { return R2 > 1 ? T * lds_ts<R1, R2>() : 1; }
 
 // One workgroup's share of a pass: instances [blk*T, blk*T + T).
 template <int R1, int R2, int NT, int T>
 __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) {
     constexpr int R = R1 * R2;
-    constexpr int TS = R + 1;
+    constexpr int RS = lds_rs<R2>(), TS = lds_ts<R1, R2>();
     const int tid = threadIdx.x;
     const u32 I0 = blk * (u32)T;
 
@@ -77,13 +84,18 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         const int t = ta % T, rb = ta / T;
         const u32 I = I0 + (u32)t;
         const bool valid = I < a.total_inst;
+        // inner twiddles w_R^(ka*rb): row rb of the table, issued before
+        // the data loads so one wait covers both
+        u64 w[R1];
+        if constexpr (R2 > 1) {
+            const u64 *twr = a.tw_inner + rb * R1;
+#pragma unroll
+            for (int ka = 1; ka < R1; ka++) w[ka] = twr[ka];
+        }
         u64 x[R1];
-        u32 b = 0, s = 0, m = 0;
+        u32 b = 0, i = 0, s = 0, m = 0;
         if (valid) {
-            b = I / a.ninst;
-            const u32 i = I - b * a.ninst;
-            s = (u32)(i % (u32)a.S);
-            m = (u32)(i / (u32)a.S);
+            locate(a, I, b, i, s, m);
             const u64 *src = a.in + (u64)b * a.bstride;
             if (a.flags & FLAG_REVERSE) {
 #pragma unroll
@@ -93,9 +105,10 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
                     x[ra] = src[s + a.S * rpos];
                 }
             } else {
-                const u64 *p = src + (u64)(s + a.S * (u64)m) + a.SMp * (u64)rb;
+                const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+                const u64 step = a.SMp * (u64)R2;
 #pragma unroll
-                for (int ra = 0; ra < R1; ra++) x[ra] = p[a.SMp * (u64)(ra * R2)];
+                for (int ra = 0; ra < R1; ra++) x[ra] = p[step * (u64)ra];
             }
         } else {
 #pragma unroll
@@ -104,14 +117,14 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         dft<R1>(x);
         if constexpr (R2 > 1) {
 #pragma unroll
-            for (int ka = 1; ka < R1; ka++) x[ka] = mont_mul(x[ka], a.tw_inner[(ka * rb) % R]);
-            u64 *row = lds + t * TS;
+            for (int ka = 1; ka < R1; ka++) x[ka] = mont_mul(x[ka], w[ka]);
+            u64 *row = lds + t * TS + rb;
 #pragma unroll
-            for (int ka = 0; ka < R1; ka++) row[ka * R2 + lds_col<R2>(rb, ka)] = x[ka];
+            for (int ka = 0; ka < R1; ka++) row[ka * RS] = x[ka];
         } else if (valid) {
             // single sub-step pass: k = ka
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * (u64)R * (u64)m;
-            store_outputs<R1>(a, dst, a.S, m, 0, 1, R, x);
+            store_outputs<R1, 1>(a, dst, a.S, m, 0, R, x);
         }
     }
     if constexpr (R2 > 1) {
@@ -134,17 +147,15 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
             }
             const u32 I = I0 + (u32)t;
             u64 z[R2];
-            const u64 *row = lds + t * TS + ka * R2;
+            const u64 *row = lds + t * TS + ka * RS;
 #pragma unroll
-            for (int rb = 0; rb < R2; rb++) z[rb] = row[lds_col<R2>(rb, ka)];
+            for (int rb = 0; rb < R2; rb++) z[rb] = row[rb];
             dft<R2>(z);
             if (I < a.total_inst) {
-                const u32 b = I / a.ninst;
-                const u32 i = I - b * a.ninst;
-                const u32 s = (u32)(i % (u32)a.S);
-                const u32 m = (u32)(i / (u32)a.S);
+                u32 b, i, s, m;
+                locate(a, I, b, i, s, m);
                 u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
-                store_outputs<R2>(a, dst, a.S * (u64)R1, m, (u32)ka, (u32)R1, (u32)R, z);
+                store_outputs<R2, R1>(a, dst, a.S * (u64)R1, m, (u32)ka, (u32)R, z);
             }
         }
     }
@@ -189,11 +200,14 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
         if (ta >= TA) break;
         const int t = ta % T, r3 = (ta / T) % R3, r2 = ta / (T * R3);
         const u32 I = I0 + (u32)t;
+        const int c = r2 * R3 + r3;
+        u64 w[R1];
+#pragma unroll
+        for (int k1 = 1; k1 < R1; k1++) w[k1] = a.tw_inner[(k1 * c) % R];
         u64 x[R1];
         if (I < a.total_inst) {
-            const u32 b = I / a.ninst;
-            const u32 i = I - b * a.ninst;
-            const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+            u32 b, i, s, m;
+            locate(a, I, b, i, s, m);
             const u64 *src = a.in + (u64)b * a.bstride;
             if (a.flags & FLAG_REVERSE) {
 #pragma unroll
@@ -211,9 +225,8 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
             for (int r1 = 0; r1 < R1; r1++) x[r1] = 0;
         }
         dft<R1>(x);
-        const int c = r2 * R3 + r3;
 #pragma unroll
-        for (int k1 = 1; k1 < R1; k1++) x[k1] = mont_mul(x[k1], a.tw_inner[(k1 * c) % R]);
+        for (int k1 = 1; k1 < R1; k1++) x[k1] = mont_mul(x[k1], w[k1]);
         u64 *row = lds + t * TS + r2 * R3 + sw<R3>(r3, r2);
 #pragma unroll
         for (int k1 = 0; k1 < R1; k1++) row[k1 * R23] = x[k1];
@@ -222,13 +235,16 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
     // ---- stage 2 (reads complete before any thread overwrites)
     {
         const int t = tid % T, r3 = (tid / T) % R3, k1 = tid / (T * R3);
+        u64 w[R2];
+#pragma unroll
+        for (int k2 = 1; k2 < R2; k2++) w[k2] = a.tw_inner[(R1 * k2 * r3) % R];
         u64 y[R2];
         const u64 *row = lds + t * TS + k1 * R23;
 #pragma unroll
         for (int r2 = 0; r2 < R2; r2++) y[r2] = row[r2 * R3 + sw<R3>(r3, r2)];
         dft<R2>(y);
 #pragma unroll
-        for (int k2 = 1; k2 < R2; k2++) y[k2] = mont_mul(y[k2], a.tw_inner[(R1 * k2 * r3) % R]);
+        for (int k2 = 1; k2 < R2; k2++) y[k2] = mont_mul(y[k2], w[k2]);
         __syncthreads();
         u64 *wrow = lds + t * TS + k1 * R23 + sw<R3>(r3, k1);
 #pragma unroll
@@ -257,12 +273,11 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
         for (int r3 = 0; r3 < R3; r3++) z[r3] = row[sw<R3>(r3, k1)];
         dft<R3>(z);
         if (I < a.total_inst) {
-            const u32 b = I / a.ninst;
-            const u32 i = I - b * a.ninst;
-            const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+            u32 b, i, s, m;
+            locate(a, I, b, i, s, m);
             const u32 kk = (u32)(k1 + R1 * k2);
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)kk + (u64)R * (u64)m);
-            store_outputs<R3>(a, dst, a.S * (u64)(R1 * R2), m, kk, (u32)(R1 * R2), (u32)R, z);
+            store_outputs<R3, R1 * R2>(a, dst, a.S * (u64)(R1 * R2), m, kk, (u32)R, z);
         }
     }
 }
@@ -332,27 +347,23 @@ static void launch_grouped(const PassArgs *descs, const u32 *starts, u32 nd, uns
 }
 
 // T: enough instances that both sub-steps have >= NT tasks, within 136 KiB LDS.
-constexpr int pick_T(int R1, int R2, int NT) {
-    int mn = R1 < R2 ? R1 : R2;
+template <int R1, int R2, int NT>
+constexpr int pick_T() {
     if (R2 == 1) return NT;
+    constexpr int mn = R1 < R2 ? R1 : R2;
     int t = (NT + mn - 1) / mn;
-    while (t > 1 && (size_t)t * (R1 * R2 + 1) * 8 > 136 * 1024) t--;
+    while (t > 1 && (size_t)t * lds_ts<R1, R2>() * 8 > 136 * 1024) t--;
     return t;
 }
 
-#define GL_VARIANT(R1, R2)                                                              \
-    {R1, R2, pick_T(R1, R2, 256), 256,                                                  \
-     (R2 > 1 ? (size_t)pick_T(R1, R2, 256) * (R1 * R2 + 1) * 8 : 8),                    \
-     &launch_pass<R1, R2, 256, pick_T(R1, R2, 256)>,                                     \
-     &launch_grouped<R1, R2, 256, pick_T(R1, R2, 256)>}
-
-// Explicit (NT, T) variants for tuning; R2 holds R2*R3 for three-stage ones.
-#define GL_VARIANT_T(R1, R2, NT, T)                                                     \
-    {R1, R2, T, NT, (size_t)(T) * ((R1) * (R2) + 1) * 8, &launch_pass<R1, R2, NT, T>,   \
-     &launch_grouped<R1, R2, NT, T>}
+#define GL_VARIANT_T(R1, R2, NT, T)                                                          \
+    {R1, R2, T, NT, (size_t)lds_words<R1, R2, T>() * 8, &launch_pass<R1, R2, NT, T>,         \
+     &launch_grouped<R1, R2, NT, T>, VK_TWO_STAGE, 1}
+#define GL_VARIANT(R1, R2) GL_VARIANT_T(R1, R2, 256, (pick_T<R1, R2, 256>()))
+// Three-stage variants: the R2 field holds R2*R3.
 #define GL_VARIANT3(R1, R2, R3, NT, T)                                                   \
     {R1, (R2) * (R3), T, NT, (size_t)(T) * ((R1) * (R2) * (R3) + 1) * 8,                 \
-     &launch_pass3<R1, R2, R3, NT, T>, &launch_pass3_grouped<R1, R2, R3, NT, T>}
+     &launch_pass3<R1, R2, R3, NT, T>, &launch_pass3_grouped<R1, R2, R3, NT, T>, VK_THREE_STAGE, R3}
 
 static const PassVariant kVariants[] = {
     // powers of two
@@ -429,8 +440,8 @@ __global__ void rader_big_gather_kernel(PassArgs a, const u32 *__restrict__ perm
     const u64 total = (u64)a.total_inst * 65536ull;
     for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
         const u32 I = (u32)(g >> 16), q = (u32)(g & 65535);
-        const u32 b = I / a.ninst, i = I - b * a.ninst;
-        const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+        u32 b, i, s, m;
+        locate(a, I, b, i, s, m);
         const u64 *src = a.in + (u64)b * a.bstride;
         auto at = [&](u64 r) {
             u64 pos = (u64)m + (u64)a.Mp * r;
@@ -466,8 +477,8 @@ __global__ void rader_big_scatter_kernel(PassArgs a, const u32 *__restrict__ per
     const u64 total = (u64)a.total_inst * 65537ull;
     for (u64 g = blockIdx.x * (u64)blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
         const u32 I = (u32)(g / 65537ull), j = (u32)(g - (u64)I * 65537ull);
-        const u32 b = I / a.ninst, i = I - b * a.ninst;
-        const u32 s = (u32)(i % (u32)a.S), m = (u32)(i / (u32)a.S);
+        u32 b, i, s, m;
+        locate(a, I, b, i, s, m);
         u64 v, k;
         if (j == 65536) {
             v = X0[I];

@@ -27,13 +27,8 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
     const bool valid = I < a.total_inst;
     const u64 *w256 = a.tw_inner, *spec = a.tw_inner + 256, *permk = a.tw_inner + 512,
               *permi = a.tw_inner + 768;
-    u32 b = 0, s = 0, m = 0;
-    if (valid) {
-        b = I / a.ninst;
-        const u32 i = I - b * a.ninst;
-        s = (u32)(i % (u32)a.S);
-        m = (u32)(i / (u32)a.S);
-    }
+    u32 b = 0, i = 0, s = 0, m = 0;
+    if (valid) locate(a, I, b, i, s, m);
     const u64 *src = a.in + (u64)b * a.bstride;
     auto load = [&](u64 r) -> u64 {
         u64 pos = (u64)m + (u64)a.Mp * r;
@@ -130,7 +125,7 @@ static void launch_rader257_grouped(const PassArgs *descs, const u32 *starts, u3
 }
 
 static const PassVariant kRader257 = {257, 1, RD_T, RD_NT, 2 * RD_T * RD_STRIDE * 8, &launch_rader257,
-                                      &launch_rader257_grouped};
+                                      &launch_rader257_grouped, VK_RADER257, 1};
 
 const PassVariant *rader257_variant() { return &kRader257; }
 

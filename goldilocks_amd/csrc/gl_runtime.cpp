@@ -3,6 +3,7 @@
 #include "gl_runtime.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <dlfcn.h>
@@ -248,13 +249,17 @@ const u64 *twiddle_matrix(u64 M, u64 R) {
 // the two-level product (one multiply per output).  It pays off while the
 // matrix stays cache resident (M up to 2^20 words = 8 MiB by default, set by
 // GOLDILOCKS_TWIDDLE_FULL_MAX); larger passes use the two-level tables.
-static u64 full_table_max() {
-    static u64 v = [] {
+static std::atomic<u64> &full_table_max_ref() {
+    static std::atomic<u64> v{[] {
         const char *e = getenv("GOLDILOCKS_TWIDDLE_FULL_MAX");
         return e ? (u64)strtoull(e, nullptr, 10) : (u64)1 << 20;
-    }();
+    }()};
     return v;
 }
+static u64 full_table_max() { return full_table_max_ref().load(); }
+u64 set_full_table_max(u64 words) { return full_table_max_ref().exchange(words); }
+
+const PassVariant *rader65537_variant();
 
 const u64 *inner_table(u64 R) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -268,6 +273,29 @@ const u64 *inner_table(u64 R) {
     }
     const u64 *d = upload(t);
     if (d) g_inner_tables[R] = d;
+    return d;
+}
+
+// Inner twiddles of a two-sub-step pass R = R1 x R2, ordered so that one
+// thread's R1 - 1 multipliers are consecutive: entry rb*R1 + ka =
+// w_R^(ka*rb) (Montgomery form).
+static std::map<std::pair<u64, u64>, const u64 *> g_inner_matrices;
+const u64 *inner_matrix(u64 R1, u64 R2) {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(R1, R2);
+    auto it = g_inner_matrices.find(key);
+    if (it != g_inner_matrices.end()) return it->second;
+    const u64 R = R1 * R2;
+    std::vector<u64> pw(R), t(R);
+    u64 w = root(R), v = 1;
+    for (u64 e = 0; e < R; e++) {
+        pw[e] = to_mont(v);
+        v = mul(v, w);
+    }
+    for (u64 rb = 0; rb < R2; rb++)
+        for (u64 ka = 0; ka < R1; ka++) t[rb * R1 + ka] = pw[(ka * rb) % R];
+    const u64 *d = upload(t);
+    if (d) g_inner_matrices[key] = d;
     return d;
 }
 
@@ -309,7 +337,8 @@ const u64 *rader257_table() {
 // Radix-65537 Rader pass: no single kernel (launch == nullptr); run_plan
 // composes gather, two batched 65536-point transforms, a pointwise product
 // with the spectrum and a scatter.
-static const PassVariant kRader65537 = {65537, 1, 1, 256, 0, nullptr, nullptr};
+static const PassVariant kRader65537 = {65537, 1, 1, 256, 0, nullptr, nullptr, VK_COMPOSITE, 1};
+const PassVariant *rader65537_variant() { return &kRader65537; }
 
 struct BigRaderTables {
     const u32 *permk = nullptr, *permi = nullptr;
@@ -625,15 +654,23 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     a.SMp = S * Mp;
     a.n_rev = n;
     u64 ninst = S * Mp, total = count * ninst;
-    if (total >= (1ull << 32) || S >= (1ull << 32))
-        return set_error(GL_E_UNSUPPORTED, "batch too large for one launch");
+    // kernels index instances with 31-bit multiply-shift division
+    if (total >= (1ull << 31)) return set_error(GL_E_UNSUPPORTED, "batch too large for one launch");
     a.ninst = (u32)ninst;
     a.total_inst = (u32)total;
     a.Mp = (u32)Mp;
-    if (pp.v->R2 > 1 || pp.v->R1 == 257) {
-        a.tw_inner = pp.v->R1 == 257 ? rader257_table() : inner_table(R);
-        if (!a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    fastdiv_params(ninst, a.ninst_mul, a.ninst_sh);
+    fastdiv_params(S, a.S_mul, a.S_sh);
+    if (pp.v->kind == VK_RADER257) {
+        a.tw_inner = rader257_table();
+    } else if (pp.v->kind == VK_THREE_STAGE) {
+        a.tw_inner = inner_table(R);
+    } else if (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1) {
+        a.tw_inner = inner_matrix((u64)pp.v->R1, (u64)pp.v->R2);
     }
+    const bool needs_inner = pp.v->kind == VK_RADER257 || pp.v->kind == VK_THREE_STAGE ||
+                             (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1);
+    if (needs_inner && !a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     if (Mp > 1) {
         const u64 *full = M <= full_table_max() ? twiddle_matrix(M, R) : nullptr;
         if (full) {
@@ -695,6 +732,7 @@ void clear_caches() {
         g_tw_allocs.clear();
         g_pow_tables.clear();
         g_inner_tables.clear();
+        g_inner_matrices.clear();
         g_full_tables.clear();
         g_rader257 = nullptr;
         g_big_rader = BigRaderTables();

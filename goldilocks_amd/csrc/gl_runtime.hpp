@@ -57,6 +57,10 @@ PowTable root_table(u64 M);
 PowTable power_table(u64 base, u64 count);
 // w_R^e for e < R (inner twiddles of a two-sub-step pass).
 const u64 *inner_table(u64 R);
+// Largest sub-problem length M whose output twiddles use a full M-word
+// table; returns the previous value.
+u64 set_full_table_max(u64 words);
+const PassVariant *rader65537_variant();
 // Tables of the radix-257 Rader pass (see gl_rader.hip).
 const u64 *rader257_table();
 

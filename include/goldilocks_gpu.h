@@ -103,6 +103,22 @@ size_t gl_variants(uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap);
  * the planner's default. */
 int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants);
 
+/* Kernel name of variant `id` (ids as for gl_plan_set; count = radix-257
+ * Rader kernel, count + 1 = radix-65537 Rader composite); NULL if unknown.
+ * Tuning/introspection aid, no reference counterpart. */
+const char *gl_variant_name(uint32_t id);
+
+/* Variant ids of the passes of the schedule for length n (the same plan
+ * gl_ntt would run). */
+int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
+
+/* Output twiddles of a pass on sub-problems of length M use one M-word table
+ * (one load and one product per output) when M <= words, else a two-level
+ * power table (w^(m*k0) * (w^(m*kstep))^j).  Results are identical either
+ * way; returns the previous threshold (default 2^20, or
+ * GOLDILOCKS_TWIDDLE_FULL_MAX). */
+uint64_t gl_set_twiddle_full_max(uint64_t words);
+
 /* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46). */
 void gl_clear_cache(void);
 
