@@ -6,7 +6,8 @@ uniform in [0, p)).  Prints one JSON line with the whole-job throughput
 oracle's port of the reference algorithm on this host's cores, bounded
 sample).  Extra workloads: --workload batch (mixed-size batched NTTs,
 batched vs one-launch-sequence-per-transform, with p50/p99 latency),
---workload commit (coset LDE + K12 Merkle commit).
+--workload commit (coset LDE + K12 Merkle commit), --workload serve (open-loop
+Poisson NTT/commit jobs through the continuous-batching server).
 """
 import argparse
 import json
@@ -27,48 +28,82 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--log-n", type=int, default=24)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve"])
+    ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
     return ap.parse_args()
 
 
-def traffic_for(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, copy-kernel calibrated), if recorded for this kernel."""
+def traffic_for(name, workload):
+    """HBM bytes per launch of kernel `name` in `workload`, measured with
+    rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE runs, each scaled by the same
+    counter's reading for a plain copy of known size); recorded in
+    profiles/traffic.json by goldilocks_amd.profiling.  None if absent."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(name)
+            rec = json.load(f).get(workload, {}).get(name)
     except (OSError, ValueError):
         return None
+    return rec["bytes_per_launch"] if rec else None
+
+
+def kernel_table(timings):
+    """[(name, ms)] -> {name: {"launches", "total_ms", "avg_us"}}"""
+    tab = {}
+    for name, ms in timings:
+        t = tab.setdefault(name, {"launches": 0, "total_ms": 0.0})
+        t["launches"] += 1
+        t["total_ms"] += ms
+    for t in tab.values():
+        t["avg_us"] = round(1e3 * t["total_ms"] / t["launches"], 2)
+        t["total_ms"] = round(t["total_ms"], 4)
+    return tab
 
 
 def cpu_baseline_ntt(n, seconds):
-    """The oracle (C port of the reference plans) on this host, 1 thread,
-    repeated whole 2^k transforms until `seconds` of CPU time."""
+    """The oracle (C port of the reference's plan tree) on this host: the
+    top Cooley-Tukey level split over up to 16 threads (as the reference's
+    ntt-par splits it over rayon), then on one thread; whole 2^k transforms,
+    each timing bounded by about seconds/2 of wall time."""
     if seconds <= 0:
         return None
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     x = oracle.fill_random(n, 1)
-    times = []
-    t_end = time.perf_counter() + seconds
-    while True:
-        t0 = time.perf_counter()
-        oracle.ntt(x)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() >= t_end or len(times) >= 50:
-            break
-    med = statistics.median(times)
-    return {"value": n / med / 1e9, "unit": "Gelem/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} forward NTTs of 2^{n.bit_length() - 1} words (median)"}
+
+    def timed(threads, budget):
+        oracle.lib().or_set_threads(threads)
+        times = []
+        t_end = time.perf_counter() + budget
+        while True:
+            t0 = time.perf_counter()
+            oracle.ntt(x)
+            times.append(time.perf_counter() - t0)
+            if time.perf_counter() >= t_end or len(times) >= 50:
+                break
+        oracle.lib().or_set_threads(1)
+        return n / statistics.median(times) / 1e9, len(times)
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    par, k_par = timed(threads, seconds / 2)
+    one, k_one = timed(1, seconds / 2)
+    return {"value": round(par, 5), "unit": "Gelem/s", "cores": threads, "kind": "port",
+            "sample": f"forward NTTs of 2^{n.bit_length() - 1} words, median of {k_par} "
+                      f"({threads} threads) and of {k_one} (1 thread)",
+            "value_1core": round(one, 5)}
 
 
 def bench_ntt(args, torch, gl):
     n = 1 << args.log_n
     x = torch.empty(n, dtype=torch.int64, device="cuda")
     gl.fill_random(x, 1)
+    # A plain copy of the same n words (8n bytes read, 8n written): the
+    # calibration dispatch for FETCH_SIZE / WRITE_SIZE when this command runs
+    # under rocprofv3 --pmc.
+    y = x.clone()
+    del y
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         gl.ntt(x)
@@ -86,12 +121,22 @@ def bench_ntt(args, torch, gl):
     per_sorted = sorted(per)
     p50 = per_sorted[len(per) // 2]
     p99 = per_sorted[min(len(per) - 1, int(0.99 * len(per)))]
+    # Kernel durations: the same steps again with every pass launch bracketed
+    # by HIP events on the launch stream (kept out of the region above so the
+    # extra event records do not perturb the step time).
+    gl.timing_collect()
+    gl.timing_enable(True)
+    for _ in range(args.steps):
+        gl.ntt(x)
+    torch.cuda.synchronize()
+    gl.timing_enable(False)
+    kernels = kernel_table(gl.timing_collect())
     plan = gl.plan_info(n)
-    passes = len(plan)
-    kname = plan[0]["kernel"] if plan else "none"
-    # Every pass reads and writes each word once: 16 n algorithmic bytes per launch.
-    per_launch_ms = ms / max(passes, 1)
-    achieved = 16.0 * n / (per_launch_ms * 1e-3) / 1e9
+    # Dominant kernel: largest share of the timed region.  Each NTT pass reads
+    # and writes every word once, so a launch moves 16 n algorithmic bytes.
+    kname = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else "none"
+    avg_ms = kernels[kname]["total_ms"] / kernels[kname]["launches"] if kernels else ms
+    achieved = 16.0 * n / (avg_ms * 1e-3) / 1e9
     gelem = n / (ms * 1e-3) / 1e9
     return {
         "metric": "Gelem/s, 2^24-point forward Goldilocks NTT (HBM-resident)",
@@ -108,12 +153,14 @@ def bench_ntt(args, torch, gl):
         "dtype": "u64",
         "data": "synthetic: seeded splitmix64 words uniform in [0,p)",
         "config": {"workload": f"forward NTT, n=2^{args.log_n}, in place, words resident in HBM",
-                   "n": n, "passes": [p["R"] for p in plan]},
+                   "n": n, "passes": [p["kernel"] for p in plan]},
         "latency_ms": {"p50": round(p50, 5), "p99": round(p99, 5)},
         "transform_hbm_frac": round(16.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_for(kname)},
+                     "traffic": traffic_for(kname, f"ntt_2^{args.log_n}"),
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": 16 * n},
     }
 
 
@@ -135,9 +182,12 @@ def main():
     elif args.workload == "batch":
         from goldilocks_amd.benchmarks import bench_batch
         out = bench_batch(args, torch, gl)
-    else:
+    elif args.workload == "commit":
         from goldilocks_amd.benchmarks import bench_commit
         out = bench_commit(args, torch, gl)
+    else:
+        from goldilocks_amd.benchmarks import bench_serve
+        out = bench_serve(args, torch, gl)
     print(json.dumps(out), flush=True)
 
 

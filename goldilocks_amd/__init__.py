@@ -79,6 +79,10 @@ def lib():
             "gl_plan_variants": (i32, [u64, vp, sz, psz]),
             "gl_variant_name": (ctypes.c_char_p, [ctypes.c_uint32]),
             "gl_set_twiddle_full_max": (u64, [u64]),
+            "gl_timing_enable": (None, [i32]),
+            "gl_set_batch_chunk_bytes": (u64, [u64]),
+            "gl_timing_pending": (sz, []),
+            "gl_timing_collect": (i32, [vp, vp, sz, psz]),
             "gl_batch_plan_create": (i32, [vp, vp, sz, i32, vp]),
             "gl_batch_plan_launch": (i32, [vp, vp]),
             "gl_batch_plan_destroy": (None, [vp]),
@@ -223,6 +227,31 @@ def variants():
     lib().gl_variants(r1, r2, t, n)
     return [{"id": i, "R1": r1[i], "R2": r2[i], "R": r1[i] * r2[i], "T": t[i],
              "name": lib().gl_variant_name(i).decode()} for i in range(n)]
+
+
+def set_batch_chunk_bytes(nbytes):
+    """Chunk size of the grouped batch path (results do not depend on it);
+    returns the previous value."""
+    return lib().gl_set_batch_chunk_bytes(int(nbytes))
+
+
+def timing_enable(on=True):
+    """Bracket every NTT pass launch with HIP events on its stream."""
+    lib().gl_timing_enable(int(bool(on)))
+
+
+def timing_collect():
+    """Wait for the passes recorded since the last call; returns a list of
+    (kernel name, milliseconds) in launch order."""
+    cap = lib().gl_timing_pending() + 64
+    ids, ms = (ctypes.c_uint32 * cap)(), (ctypes.c_float * cap)()
+    cnt = ctypes.c_size_t(0)
+    _check(lib().gl_timing_collect(ids, ms, cap, ctypes.byref(cnt)))
+    out = []
+    for i in range(min(cnt.value, cap)):
+        name = lib().gl_variant_name(ids[i])
+        out.append((name.decode() if name else "?", ms[i]))
+    return out
 
 
 def set_twiddle_full_max(words):

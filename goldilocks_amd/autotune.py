@@ -101,8 +101,12 @@ TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx95
 
 def write_tune_file(entries, path=TUNE_FILE):
     """Write schedules in the text format the library loads at init:
-    "n R1xR2/T R1xR2/T ..." per line (merged with existing lines)."""
+    "n R1xR2/T R1xR2/T ..." per line (merged with existing lines).  R2 is the
+    variant's R2 field (R2*R3 for three-stage kernels); (R1, R2, T) names one
+    compiled variant."""
     vs = {v["id"]: v for v in gl.variants()}
+    keys = [(v["R1"], v["R2"], v["T"]) for v in vs.values()]
+    assert len(keys) == len(set(keys)), "tune-file keys must name one variant each"
     lines = {}
     if os.path.exists(path):
         for line in open(path):
@@ -160,6 +164,7 @@ if __name__ == "__main__":
         best, best_ms, base_ms = tune(n)
         # keep the default when the win is within timing noise
         found[n] = best if best is not None and best_ms < 0.97 * base_ms else None
-    save(found)
-    if write or out:
-        write_tune_file(found, out or TUNE_FILE)
+        # write after every length so a cut-off run keeps what it measured
+        save({n: found[n]})
+        if write or out:
+            write_tune_file({n: found[n]}, out or TUNE_FILE)

@@ -3,13 +3,16 @@ import random
 import time
 
 
-def mixed_sizes(count, seed, lo=10, hi=20, factors=(1, 3, 5, 15, 17)):
+def mixed_sizes(count, seed, lo=10, hi=20, factors=(1, 3, 5, 15, 17, 257)):
     """Lengths 2^k * f, k uniform in [lo, hi], f from `factors`."""
     rng = random.Random(seed)
     return [(1 << rng.randint(lo, hi)) * rng.choice(factors) for _ in range(count)]
 
 
 def bench_batch(args, torch, gl):
+    """1024 forward transforms, n = 2^k * f (k uniform in [10, 20], f from
+    {1, 3, 5, 15, 17, 257}), device-resident: grouped launches vs one launch
+    sequence per transform; per-batch latency p50/p99 over the repetitions."""
     sizes = mixed_sizes(1024, 7)
     total = sum(sizes)
     bufs = [torch.empty(n, dtype=torch.int64, device="cuda") for n in sizes]
@@ -21,10 +24,10 @@ def bench_batch(args, torch, gl):
         gl.ntt_batch_mixed(bufs, batched=batched)
 
     res = {}
+    reps = max(3, min(args.steps, 12))
     for batched in (True, False):
         run(batched)
         torch.cuda.synchronize()
-        reps = max(1, min(args.steps, 5))
         times = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -34,21 +37,26 @@ def bench_batch(args, torch, gl):
             torch.cuda.synchronize()
             times.append(e0.elapsed_time(e1))
         times.sort()
+        med = times[len(times) // 2]
         res["batched" if batched else "unbatched"] = {
-            "ms": times[len(times) // 2], "gelem_s": total / (times[len(times) // 2] * 1e-3) / 1e9}
+            "ms_p50": round(med, 3), "ms_p99": round(times[min(len(times) - 1, int(0.99 * len(times)))], 3),
+            "gelem_s": round(total / (med * 1e-3) / 1e9, 3)}
     return {
-        "metric": "Gelem/s, mixed-size batched NTT (1024 transforms)",
-        "value": round(res["batched"]["gelem_s"], 3),
+        "metric": "Gelem/s, mixed-size batched NTT (1024 transforms); p50/p99 batch latency",
+        "value": res["batched"]["gelem_s"],
         "unit": "Gelem/s",
         "n_gpus": 1,
-        "ms_per_step": round(res["batched"]["ms"], 3),
+        "steps": reps,
+        "ms_per_step": res["batched"]["ms_p50"],
         "higher_is_better": True,
         "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": "1024 forward NTTs, n = 2^k * f, k in [10,20], f in {1,3,5,15,17}",
-                   "elements": total},
+        "data": "synthetic: seeded uniform [0,p) words, device-resident",
+        "config": {"workload": "1024 forward NTTs, n = 2^k * f, k uniform in [10,20], f in {1,3,5,15,17,257}",
+                   "elements": total, "bytes": 8 * total},
+        "latency_ms": {"p50": res["batched"]["ms_p50"], "p99": res["batched"]["ms_p99"]},
+        "batched": res["batched"],
         "unbatched": res["unbatched"],
-        "speedup_vs_unbatched": round(res["unbatched"]["ms"] / res["batched"]["ms"], 3),
+        "speedup_vs_unbatched": round(res["unbatched"]["ms_p50"] / res["batched"]["ms_p50"], 3),
     }
 
 
@@ -81,4 +89,103 @@ def bench_commit(args, torch, gl):
         "data": "synthetic",
         "config": {"workload": "column coset LDE (2^21-point NTTs) + K12 Merkle tree over 1 GiB",
                    "rows": rows, "cols": cols, "blowup": blowup},
+    }
+
+
+SERVE_FACTORS = (1, 3, 5, 15, 17, 257)
+
+
+def serve_jobs(count, seed, max_elems=1 << 24):
+    """Seeded job list: ("ntt", n, cols) or ("commit", rows, cols).
+    n = 2^k * f with k in [12, 22], f from SERVE_FACTORS; 1-64 columns;
+    n * cols capped at max_elems by lowering k, then cols."""
+    rng = random.Random(seed)
+    jobs = []
+    for _ in range(count):
+        cols = rng.randint(1, 64)
+        if rng.random() < 0.25:
+            k = rng.randint(12, 18)
+            while (1 << k) * 2 * cols > max_elems:
+                cols = max(1, cols // 2)
+            jobs.append(("commit", 1 << k, cols))
+        else:
+            f = rng.choice(SERVE_FACTORS)
+            k = rng.randint(12, 22)
+            while k > 12 and (1 << k) * f > max_elems:
+                k -= 1
+            n = (1 << k) * f
+            while n * cols > max_elems and cols > 1:
+                cols //= 2
+            jobs.append(("ntt", n, cols))
+    return jobs
+
+
+def bench_serve(args, torch, gl):
+    """Open-loop Poisson arrivals of NTT and commit jobs into the
+    continuous-batching server; p50/p99 latency, throughput, admissions."""
+    count = 1000
+    jobs = serve_jobs(count, 11)
+    elems = [n * c if kind == "ntt" else 2 * n * c for kind, n, c in jobs]
+    # Offered load: about 60% of a 60 Gelem/s device, as a fixed QPS.
+    qps = args.qps or 0.6 * 60e9 / (sum(elems) / count)
+    rng = random.Random(13)
+    t, arrivals = 0.0, []
+    for _ in jobs:
+        t += rng.expovariate(qps)
+        arrivals.append(t)
+    bufs = []
+    for i, (kind, n, c) in enumerate(jobs):
+        if kind == "ntt":
+            x = torch.empty(n * c, dtype=torch.int64, device="cuda")
+            gl.fill_random(x, 100 + i)
+            bufs.append((x,))
+        else:
+            m = torch.empty((n, c), dtype=torch.int64, device="cuda")
+            gl.fill_random(m, 100 + i)
+            lde = torch.empty((2 * n, c), dtype=torch.int64, device="cuda")
+            layers = torch.empty(gl.merkle_tree_size(2 * n * c * 8), dtype=torch.uint8, device="cuda")
+            bufs.append((m, lde, layers))
+    torch.cuda.synchronize()
+    budget = 32 << 30
+    with gl.Server(hbm_budget_bytes=budget, slots=4) as srv:
+        # warm the plans and twiddle tables of every shape once
+        seen = set()
+        for (kind, n, c), b in zip(jobs, bufs):
+            if (kind, n) in seen:
+                continue
+            seen.add((kind, n))
+            tk = srv.submit_ntt(b[0], n) if kind == "ntt" else srv.submit_commit(b[0], 2, b[1], b[2])
+            srv.wait(tk)
+        base = srv.stats()
+        tickets = []
+        t0 = time.perf_counter()
+        for (kind, n, c), b, a in zip(jobs, bufs, arrivals):
+            delay = t0 + a - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit_ntt(b[0], n) if kind == "ntt" else srv.submit_commit(b[0], 2, b[1], b[2]))
+        lat = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat.sort()
+    total = sum(elems)
+    return {
+        "metric": "Gelem/s and p50/p99 latency, served mixed NTT/commit jobs (open-loop Poisson)",
+        "value": round(total / wall / 1e9, 3),
+        "unit": "Gelem/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u64",
+        "data": "synthetic: seeded uniform [0,p) words, device-resident buffers",
+        "config": {"workload": f"{count} jobs, 75% NTT (n = 2^k*f, k in [12,22], f in {list(SERVE_FACTORS)}, "
+                               "1-64 transforms) and 25% commits (2^12..2^18 rows x 1-64 cols, blowup 2), "
+                               "n*cols <= 2^24", "qps": round(qps, 1), "slots": 4,
+                   "hbm_budget_bytes": budget},
+        "latency_ms": {"p50": round(lat[len(lat) // 2], 4), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 4),
+                       "max": round(lat[-1], 4)},
+        "wall_s": round(wall, 4),
+        "elements": total,
+        "admitted": int(st["completed"] - base["completed"]),
+        "rejected": int(st["rejected"] - base["rejected"]),
+        "peak_bytes": int(st["peak_bytes"]),
     }

@@ -220,6 +220,14 @@ int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
 
 uint64_t gl_set_twiddle_full_max(uint64_t words) { return set_full_table_max(words); }
 
+void gl_timing_enable(int on) { timing_enable(on != 0); }
+
+size_t gl_timing_pending(void) { return timing_pending(); }
+
+int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *count) {
+    return timing_collect(variant_ids, ms, cap, count);
+}
+
 int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants) {
     if (nvariants && !variant_ids) return set_error(GL_E_ARG, "null pointer");
     return set_plan(n, variant_ids, nvariants);

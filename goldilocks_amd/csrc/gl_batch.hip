@@ -1,13 +1,19 @@
 // Mixed-size batches: many transforms of different lengths and radices.
 //
-// Batched path: level-synchronous grouped launches.  For pass index L, all
-// transforms whose plan has an L-th pass with the same kernel variant share
-// one launch whose blocks are split across per-transform descriptors, so the
-// launch count depends on the distinct (level, variant) pairs, not on the
-// number of transforms.  gl_batch_plan_* keeps the descriptors resident and
+// Batched path: level-synchronous grouped launches over cache-sized chunks.
+// Transforms are packed in order into chunks of at most chunk_words() words;
+// within a chunk, for pass index L, all transforms whose plan has an L-th
+// pass with the same kernel variant share one launch whose blocks are split
+// across per-transform descriptors, so the launch count depends on the
+// distinct (level, variant) pairs, not on the number of transforms, while a
+// chunk's words stay in the last-level cache from one pass to the next.
+// Transforms larger than a chunk (which fill the device by themselves) and
+// radix-65537 ones run on their own.  gl_batch_plan_* keeps the descriptors resident and
 // replays the launch sequence as a captured hipGraph.  The unbatched baseline
 // issues one launch sequence per transform.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -27,9 +33,10 @@ struct Group {
 
 struct Schedule {
     std::vector<const Plan *> plans;
-    std::vector<size_t> composite;
-    std::vector<u64> tmp_off;
-    u64 tmp_words = 0;
+    std::vector<size_t> alone;                // run as single transforms (gl_ntt_dev)
+    std::vector<std::vector<size_t>> chunks;  // grouped level by level
+    std::vector<u64> tmp_off;                 // scratch offset; restarts in every chunk
+    u64 tmp_words = 0;                        // largest chunk's scratch
     std::vector<Group> groups;
     std::vector<PassArgs> descs;
     std::vector<u32> starts;
@@ -37,67 +44,95 @@ struct Schedule {
     size_t meta_bytes() const { return desc_bytes() + starts.size() * sizeof(u32); }
 };
 
+// Transforms are grouped in chunks of at most this many words, so that a
+// chunk's data and scratch stay resident in the device's last-level cache
+// between its passes; a transform larger than a chunk runs by itself, where
+// its own passes get that reuse.  GOLDILOCKS_BATCH_CHUNK_BYTES overrides.
+std::atomic<u64> &chunk_bytes_ref() {
+    static std::atomic<u64> b{[] {
+        const char *e = getenv("GOLDILOCKS_BATCH_CHUNK_BYTES");
+        return e ? (u64)strtoull(e, nullptr, 10) : (u64)64 << 20;
+    }()};
+    return b;
+}
+u64 chunk_words() {
+    u64 w = chunk_bytes_ref().load() / 8;
+    return w ? w : 1;
+}
+
 int plan_batch(uint64_t *const *d, const size_t *n, size_t k, bool inverse, Schedule &sc) {
     sc.plans.assign(k, nullptr);
     sc.tmp_off.assign(k, 0);
     sc.tmp_words = 0;
+    const u64 cw = chunk_words();
+    u64 cur = 0;
     int rc;
     for (size_t i = 0; i < k; i++) {
         if (inverse && n[i] <= 1) continue;
         if ((rc = get_plan(n[i], &sc.plans[i]))) return rc;
-        if (plan_is_composite(*sc.plans[i])) {
-            sc.composite.push_back(i);  // not groupable: run on its own
+        if (sc.plans[i]->passes.empty()) {
             sc.plans[i] = nullptr;
             continue;
         }
-        if (!d[i] && !sc.plans[i]->passes.empty()) return set_error(GL_E_ARG, "null device pointer");
+        if (!d[i]) return set_error(GL_E_ARG, "null device pointer");
+        if (plan_is_composite(*sc.plans[i]) || n[i] > cw) {
+            sc.alone.push_back(i);  // not groupable, or big enough to fill the device alone
+            continue;
+        }
+        const u64 w = (n[i] + 31) / 32 * 32;
+        if (sc.chunks.empty() || cur + w > cw) {
+            sc.chunks.emplace_back();
+            cur = 0;
+        }
+        sc.chunks.back().push_back(i);
         if (sc.plans[i]->passes.size() > 1) {
-            sc.tmp_off[i] = sc.tmp_words;
-            sc.tmp_words += (n[i] + 31) / 32 * 32;
+            sc.tmp_off[i] = cur;
+            cur += w;
+            sc.tmp_words = std::max(sc.tmp_words, cur);
         }
     }
     return GL_OK;
 }
 
-int build_groups(uint64_t *const *d, const size_t *n, size_t k, bool inverse, u64 *tmp, Schedule &sc) {
-    size_t levels = 0;
-    std::vector<PassCursor> cur;
-    cur.reserve(k);
-    for (size_t i = 0; i < k; i++) {
-        PassCursor c{sc.plans[i], d[i], tmp ? tmp + sc.tmp_off[i] : nullptr, 1, 1, n[i], inverse};
-        if (sc.plans[i]) {
+int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Schedule &sc) {
+    int rc;
+    for (const std::vector<size_t> &chunk : sc.chunks) {
+        size_t levels = 0;
+        std::vector<PassCursor> cur;
+        cur.reserve(chunk.size());
+        for (size_t i : chunk) {
+            PassCursor c{sc.plans[i], d[i], tmp ? tmp + sc.tmp_off[i] : nullptr, 1, 1, n[i], inverse};
             c.begin();
             levels = std::max(levels, sc.plans[i]->passes.size());
+            cur.push_back(c);
         }
-        cur.push_back(c);
-    }
-    int rc;
-    for (size_t L = 0; L < levels; L++) {
-        std::vector<std::pair<const PassVariant *, size_t>> items;  // (variant, transform)
-        for (size_t i = 0; i < k; i++)
-            if (sc.plans[i] && L < sc.plans[i]->passes.size()) items.emplace_back(sc.plans[i]->passes[L].v, i);
-        std::stable_sort(items.begin(), items.end(),
-                         [](const std::pair<const PassVariant *, size_t> &x,
-                            const std::pair<const PassVariant *, size_t> &y) { return x.first < y.first; });
-        for (size_t a = 0; a < items.size();) {
-            size_t b = a;
-            while (b < items.size() && items[b].first == items[a].first) b++;
-            Group g{items[a].first, sc.descs.size(), b - a, sc.starts.size(), 0};
-            u32 blocks = 0;
-            for (size_t j = a; j < b; j++) {
-                PassArgs pa;
-                unsigned grid = 0;
-                if ((rc = cur[items[j].second].next(pa, grid))) return rc;
-                sc.descs.push_back(pa);
+        for (size_t L = 0; L < levels; L++) {
+            std::vector<std::pair<const PassVariant *, size_t>> items;  // (variant, position in chunk)
+            for (size_t j = 0; j < chunk.size(); j++)
+                if (L < sc.plans[chunk[j]]->passes.size()) items.emplace_back(sc.plans[chunk[j]]->passes[L].v, j);
+            std::stable_sort(items.begin(), items.end(),
+                             [](const std::pair<const PassVariant *, size_t> &x,
+                                const std::pair<const PassVariant *, size_t> &y) { return x.first < y.first; });
+            for (size_t a = 0; a < items.size();) {
+                size_t b = a;
+                while (b < items.size() && items[b].first == items[a].first) b++;
+                Group g{items[a].first, sc.descs.size(), b - a, sc.starts.size(), 0};
+                u32 blocks = 0;
+                for (size_t j = a; j < b; j++) {
+                    PassArgs pa;
+                    unsigned grid = 0;
+                    if ((rc = cur[items[j].second].next(pa, grid))) return rc;
+                    sc.descs.push_back(pa);
+                    sc.starts.push_back(blocks);
+                    if ((u64)blocks + grid >= (1ull << 31))
+                        return set_error(GL_E_UNSUPPORTED, "grouped launch too large");
+                    blocks += grid;
+                }
                 sc.starts.push_back(blocks);
-                if ((u64)blocks + grid >= (1ull << 31))
-                    return set_error(GL_E_UNSUPPORTED, "grouped launch too large");
-                blocks += grid;
+                g.blocks = blocks;
+                sc.groups.push_back(g);
+                a = b;
             }
-            sc.starts.push_back(blocks);
-            g.blocks = blocks;
-            sc.groups.push_back(g);
-            a = b;
         }
     }
     return GL_OK;
@@ -109,18 +144,23 @@ void pack_meta(const Schedule &sc, std::vector<unsigned char> &host) {
     std::memcpy(host.data() + sc.desc_bytes(), sc.starts.data(), sc.starts.size() * sizeof(u32));
 }
 
-int launch_groups(const Schedule &sc, const void *meta, hipStream_t s) {
+// timed: bracket launches for gl_timing_collect (never inside a graph capture)
+int launch_groups(const Schedule &sc, const void *meta, hipStream_t s, bool timed) {
     const PassArgs *ddesc = (const PassArgs *)meta;
     const u32 *dstarts = (const u32 *)((const unsigned char *)meta + sc.desc_bytes());
     for (const Group &g : sc.groups) {
         if (g.blocks == 0) continue;
+        hipEvent_t e0 = timed ? timing_begin(s) : nullptr;
         g.v->launch_grouped(ddesc + g.desc_off, dstarts + g.starts_off, (u32)g.nd, g.blocks, s);
         GL_HIP(hipGetLastError());
+        timing_end(g.v, e0, s);
     }
     return GL_OK;
 }
 
 }  // namespace
+
+extern "C" uint64_t gl_set_batch_chunk_bytes(uint64_t bytes) { return chunk_bytes_ref().exchange(bytes); }
 
 extern "C" int gl_ntt_mixed_dev_unbatched(uint64_t *const *d, const size_t *n, size_t k, int inverse,
                                           void *stream) {
@@ -140,18 +180,20 @@ extern "C" int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, i
     hipStream_t s = (hipStream_t)stream;
     Schedule sc;
     if ((rc = plan_batch(d, n, k, inverse != 0, sc))) return rc;
-    for (size_t i : sc.composite)
+    // stand-alone transforms first: they may grow this stream's scratch, which
+    // the grouped launches below then reuse
+    for (size_t i : sc.alone)
         if ((rc = gl_ntt_dev(d[i], n[i], 1, inverse, stream))) return rc;
     u64 *tmp = stream_scratch(s, sc.tmp_words, 0);
     if (sc.tmp_words && !tmp) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    if ((rc = build_groups(d, n, k, inverse != 0, tmp, sc))) return rc;
+    if ((rc = build_groups(d, n, inverse != 0, tmp, sc))) return rc;
     if (sc.groups.empty()) return GL_OK;
     u64 *meta = stream_scratch(s, sc.meta_bytes() / 8 + 64, 1);
     if (!meta) return set_error(GL_E_NOMEM, "descriptor allocation failed");
     std::vector<unsigned char> host;
     pack_meta(sc, host);
     GL_HIP(hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s));
-    if ((rc = launch_groups(sc, meta, s))) return rc;
+    if ((rc = launch_groups(sc, meta, s, true))) return rc;
     // `host` is released on return: wait until the copy has consumed it.
     GL_HIP(hipStreamSynchronize(s));
     return GL_OK;
@@ -215,18 +257,39 @@ extern "C" int gl_batch_plan_create(uint64_t *const *d, const size_t *n, size_t 
         gl_batch_plan_destroy(p);
         return rc;
     }
-    if (!sc.composite.empty()) {
-        gl_batch_plan_destroy(p);
-        return set_error(GL_E_UNSUPPORTED, "batch plans cannot capture lengths with a 65537 factor");
+    // stand-alone transforms are captured too, one after another, sharing the
+    // plan's scratch with the grouped chunks
+    u64 tmp_words = sc.tmp_words;
+    for (size_t i : sc.alone) {
+        if (plan_is_composite(*sc.plans[i])) {
+            gl_batch_plan_destroy(p);
+            return set_error(GL_E_UNSUPPORTED, "batch plans cannot capture lengths with a 65537 factor");
+        }
+        tmp_words = std::max<u64>(tmp_words, plan_tmp_words(*sc.plans[i], 1, n[i]));
     }
-    if (sc.tmp_words && hipMalloc(&p->tmp, sc.tmp_words * 8) != hipSuccess) {
+    if (tmp_words && hipMalloc(&p->tmp, tmp_words * 8) != hipSuccess) {
         (void)hipGetLastError();
         gl_batch_plan_destroy(p);
         return set_error(GL_E_NOMEM, "batch plan scratch allocation failed");
     }
-    if ((rc = build_groups(d, n, k, inverse != 0, (u64 *)p->tmp, sc))) {
+    if ((rc = build_groups(d, n, inverse != 0, (u64 *)p->tmp, sc))) {
         gl_batch_plan_destroy(p);
         return rc;
+    }
+    // build the stand-alone transforms' twiddle tables now: table setup
+    // synchronizes the device, which a capturing thread may not do
+    for (size_t i : sc.alone) {
+        PassCursor c{sc.plans[i], d[i], (u64 *)p->tmp, 1, 1, n[i], inverse != 0};
+        c.begin();
+        while (!c.done() && !rc) {
+            PassArgs pa;
+            unsigned grid = 0;
+            rc = c.next(pa, grid);
+        }
+        if (rc) {
+            gl_batch_plan_destroy(p);
+            return rc;
+        }
     }
     std::vector<unsigned char> host;
     pack_meta(sc, host);
@@ -241,7 +304,9 @@ extern "C" int gl_batch_plan_create(uint64_t *const *d, const size_t *n, size_t 
         gl_batch_plan_destroy(p);
         return rc;
     }
-    rc = launch_groups(sc, p->meta, cs);
+    for (size_t i : sc.alone)
+        if (!rc) rc = run_plan(*sc.plans[i], d[i], (u64 *)p->tmp, 1, 1, n[i], inverse != 0, cs);
+    if (!rc) rc = launch_groups(sc, p->meta, cs, false);
     e = hipStreamEndCapture(cs, &p->graph);
     if (!rc && e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     (void)hipStreamDestroy(cs);

@@ -698,6 +698,104 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     return GL_OK;
 }
 
+// ---------------------------------------------------------------- pass timing
+// While enabled, every pass launch is bracketed by HIP events on its stream;
+// timing_collect() waits for them and returns (variant id, milliseconds).
+namespace {
+struct TimingRec {
+    u32 variant;
+    hipEvent_t e0, e1;
+};
+std::mutex g_tm_mu;
+std::atomic<bool> g_tm_on{false};
+std::vector<TimingRec> g_tm_recs;
+std::vector<hipEvent_t> g_tm_pool;
+
+hipEvent_t tm_event() {
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    if (!g_tm_pool.empty()) {
+        hipEvent_t e = g_tm_pool.back();
+        g_tm_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+}  // namespace
+
+u32 variant_id(const PassVariant *v) {
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    if (v >= vs && v < vs + cnt) return (u32)(v - vs);
+    if (v == rader257_variant()) return (u32)cnt;
+    return (u32)cnt + 1;
+}
+
+void timing_enable(bool on) { g_tm_on.store(on); }
+
+// Brackets one launch: call with e0 = timing_begin(s) before and
+// timing_end(v, e0, s) after.
+hipEvent_t timing_begin(hipStream_t s) {
+    if (!g_tm_on.load()) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (cs != hipStreamCaptureStatusNone) return nullptr;  // no timing inside a graph capture
+    hipEvent_t e = tm_event();
+    if (e && hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+void timing_end(const PassVariant *v, hipEvent_t e0, hipStream_t s) {
+    if (!e0) return;
+    hipEvent_t e1 = tm_event();
+    if (!e1 || hipEventRecord(e1, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    g_tm_recs.push_back({variant_id(v), e0, e1});
+}
+
+size_t timing_pending() {
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    return g_tm_recs.size();
+}
+
+int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count) {
+    std::vector<TimingRec> recs;
+    {
+        std::lock_guard<std::mutex> lk(g_tm_mu);
+        recs.swap(g_tm_recs);
+    }
+    if (count) *count = recs.size();
+    int rc = GL_OK;
+    for (size_t i = 0; i < recs.size(); i++) {
+        float t = 0.f;
+        hipError_t e = hipEventSynchronize(recs[i].e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, recs[i].e0, recs[i].e1);
+        if (e != hipSuccess && !rc) rc = hip_error(e, "pass timing");
+        if (i < cap) {
+            if (ids) ids[i] = recs[i].variant;
+            if (ms) ms[i] = t;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    for (const TimingRec &r : recs) {
+        g_tm_pool.push_back(r.e0);
+        g_tm_pool.push_back(r.e1);
+    }
+    return rc;
+}
+
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
              hipStream_t stream, bool scale_out) {
     if (plan.passes.empty() || count == 0) return GL_OK;
@@ -710,12 +808,14 @@ int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstri
         unsigned grid = 0;
         int rc = c.next(a, grid);
         if (rc) return rc;
+        hipEvent_t e0 = timing_begin(stream);
         if (!v->launch) {
             if ((rc = run_rader_big(a, stream))) return rc;
-            continue;
+        } else {
+            v->launch(a, grid, stream);
+            GL_HIP(hipGetLastError());
         }
-        v->launch(a, grid, stream);
-        GL_HIP(hipGetLastError());
+        timing_end(v, e0, stream);
     }
     return GL_OK;
 }

@@ -111,6 +111,14 @@ size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
 
 void clear_caches();
 
+// Per-pass timing (gl_timing_enable / gl_timing_collect).
+u32 variant_id(const PassVariant *v);
+void timing_enable(bool on);
+hipEvent_t timing_begin(hipStream_t s);
+void timing_end(const PassVariant *v, hipEvent_t e0, hipStream_t s);
+size_t timing_pending();
+int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count);
+
 // Commit pieces shared by the C ABI and the server (gl_api.cpp).
 int commit_check(size_t rows, size_t cols, unsigned blowup);
 size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup);

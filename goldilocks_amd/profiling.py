@@ -1,0 +1,121 @@
+"""Summaries of rocprofv3 runs of bench.py (kernel time and HBM traffic).
+
+Traffic per launch comes from two counter runs of the same command,
+`rocprofv3 --pmc FETCH_SIZE` and `rocprofv3 --pmc WRITE_SIZE` (KiB per
+dispatch).  Each counter is scaled by its reading for a plain device copy of
+known size that bench.py issues (torch clone of the n-word input: 8n bytes
+read, 8n written), so the result is in bytes as the copy defines them.
+
+    python -m goldilocks_amd.profiling --workload ntt_2^24 --copy-bytes 134217728 \\
+        --stats <kernel_stats.csv> --fetch <counter_collection.csv> \\
+        --write <counter_collection.csv> --json profiles/traffic.json --md out.md
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+
+
+def per_dispatch(path, counter):
+    """{dispatch id: (kernel name, value)} for one counter (summed over dimensions)."""
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            d = int(r["Dispatch_Id"])
+            name, v = out.get(d, (r["Kernel_Name"], 0.0))
+            out[d] = (name, v + float(r["Counter_Value"]))
+    return out
+
+
+def calibrated(path, counter, copy_bytes):
+    """Per-kernel average bytes per dispatch, scaled so that the largest copy
+    dispatch reads (or writes) copy_bytes."""
+    disp = per_dispatch(path, counter)
+    copies = [v for name, v in disp.values() if "copyBuffer" in name]
+    if not copies or max(copies) <= 0:
+        raise SystemExit(f"{path}: no copy dispatch to calibrate {counter}")
+    scale = copy_bytes / max(copies)  # bytes per counter unit
+    acc = collections.defaultdict(list)
+    for name, v in disp.values():
+        acc[name].append(v * scale)
+    return scale, {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def short(name):
+    name = name.replace("(gl::PassArgs)", "").replace("void ", "").replace("gl::", "")
+    return name.strip()
+
+
+def kernel_stats(path):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]) / 1e3,
+                         float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+    return sorted(rows, key=lambda x: -x[2])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", required=True)
+    ap.add_argument("--copy-bytes", type=int, required=True)
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--algorithmic-bytes", type=int, default=None,
+                    help="algorithmic bytes per launch of the pass kernels (16 n for an NTT pass)")
+    ap.add_argument("--json", required=True)
+    ap.add_argument("--md", required=True)
+    ap.add_argument("--command", default="")
+    args = ap.parse_args()
+    fs, fetch = calibrated(args.fetch, "FETCH_SIZE", args.copy_bytes)
+    ws, write = calibrated(args.write, "WRITE_SIZE", args.copy_bytes)
+    stats = kernel_stats(args.stats)
+    data = {}
+    if os.path.exists(args.json):
+        with open(args.json) as f:
+            data = json.load(f)
+    rec = {}
+    for name in fetch:
+        if name not in write or "copyBuffer" in name:
+            continue
+        fb, fn = fetch[name]
+        wb, wn = write[name]
+        rec[short(name)] = {"bytes_per_launch": round(fb + wb), "fetch_bytes": round(fb),
+                            "write_bytes": round(wb), "dispatches": [fn, wn]}
+    data[args.workload] = rec
+    data.setdefault("_calibration", {})[args.workload] = {
+        "FETCH_SIZE_bytes_per_unit": fs, "WRITE_SIZE_bytes_per_unit": ws, "copy_bytes": args.copy_bytes}
+    with open(args.json, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    with open(args.md, "w") as f:
+        f.write(f"# rocprofv3 summary: {args.workload}\n\n")
+        if args.command:
+            f.write("Commands (repo root, one MI355X):\n\n")
+            for c in args.command.split(";;"):
+                f.write(f"    {c.strip()}\n")
+            f.write("\n")
+        f.write("Kernel time (`--kernel-trace --stats`, kernel_stats.csv):\n\n")
+        f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
+        for name, calls, tot, avg, pct in stats[:12]:
+            f.write(f"| `{short(name)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |\n")
+        f.write(f"\nHBM traffic per launch (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`; one counter unit = "
+                f"{fs:.1f} B read / {ws:.1f} B written, from the {args.copy_bytes}-byte copy):\n\n")
+        f.write("| kernel | launches | fetch MiB | write MiB | total MiB |")
+        if args.algorithmic_bytes:
+            f.write(" / algorithmic |")
+        f.write("\n|---|---|---|---|---|" + ("---|" if args.algorithmic_bytes else "") + "\n")
+        for name, r in sorted(rec.items(), key=lambda kv: -kv[1]["bytes_per_launch"]):
+            line = (f"| `{name}` | {r['dispatches'][0]} | {r['fetch_bytes'] / 2**20:.1f} | "
+                    f"{r['write_bytes'] / 2**20:.1f} | {r['bytes_per_launch'] / 2**20:.1f} |")
+            if args.algorithmic_bytes:
+                line += f" {r['bytes_per_launch'] / args.algorithmic_bytes:.3f} |"
+            f.write(line + "\n")
+    print(json.dumps(rec, indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -76,8 +76,17 @@ int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream);
  * c, c + cols, c + 2*cols, ...; `count` such matrices back to back. */
 int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int inverse, void *stream);
 
-/* Mixed-size batch on device buffers d[i] of n[i] words. */
+/* Mixed-size batch on device buffers d[i] of n[i] words.  Transforms are
+ * packed in order into chunks of at most gl_set_batch_chunk_bytes bytes
+ * (default 64 MiB, or GOLDILOCKS_BATCH_CHUNK_BYTES); each chunk runs as
+ * grouped launches, one per (pass level, kernel variant), so its words stay
+ * cache resident between passes.  Larger transforms and radix-65537 ones run
+ * on their own.  Returns after the stream has consumed the launch data. */
 int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, int inverse, void *stream);
+
+/* Chunk size of the grouped batch path; returns the previous value.  Results
+ * do not depend on it. */
+uint64_t gl_set_batch_chunk_bytes(uint64_t bytes);
 
 /* The same batch, one launch sequence per transform (the unbatched
  * baseline the batched path is measured against). */
@@ -118,6 +127,18 @@ int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
  * way; returns the previous threshold (default 2^20, or
  * GOLDILOCKS_TWIDDLE_FULL_MAX). */
 uint64_t gl_set_twiddle_full_max(uint64_t words);
+
+/* Pass timing (measurement aid, no reference counterpart).  While enabled,
+ * every NTT pass launch (gl_ntt*, gl_ntt_cols_dev, the commit's transforms
+ * and grouped batch launches) is bracketed by HIP events on its stream.
+ * gl_timing_pending = passes recorded since the last collect.
+ * gl_timing_collect waits for them and returns, in launch order, the variant
+ * id (as for gl_variant_name) and duration in ms of up to cap of them;
+ * *count = number recorded (all are released).  Launches made while a
+ * stream is being captured into a graph are not timed. */
+void gl_timing_enable(int on);
+size_t gl_timing_pending(void);
+int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *count);
 
 /* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46). */
 void gl_clear_cache(void);

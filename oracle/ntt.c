@@ -205,22 +205,106 @@ static Plan *strategy(size_t n) {
 
 static void exec(const Plan *p, uint64_t *v);
 
-/* rows x cols -> cols x rows */
-static void transpose(const uint64_t *src, uint64_t *dst, size_t rows, size_t cols) {
-    const size_t B = 16;
-    for (size_t i0 = 0; i0 < rows; i0 += B)
-        for (size_t j0 = 0; j0 < cols; j0 += B) {
-            size_t i1 = i0 + B < rows ? i0 + B : rows, j1 = j0 + B < cols ? j0 + B : cols;
-            for (size_t i = i0; i < i1; i++)
-                for (size_t j = j0; j < j1; j++) dst[j * rows + i] = src[i * cols + j];
-        }
-}
-
 static uint64_t *scratch(size_t n, uint64_t *stack, size_t cap) {
     return n <= cap ? stack : (uint64_t *)malloc(n * sizeof(uint64_t));
 }
 static void release(uint64_t *s, uint64_t *stack) {
     if (s != stack) free(s);
+}
+
+/* Parallel loop for the top level of one large transform: the reference's
+ * ntt-par runs the inner NTTs of its Cooley-Tukey step as rayon par_chunks
+ * (cooley_tukey.rs:100-112); here g_threads pthreads split the index range.
+ * Nested levels stay serial. */
+static __thread int tl_in_parallel;
+
+typedef struct {
+    void (*fn)(void *ctx, size_t i);
+    void *ctx;
+    size_t begin, end;
+} Range;
+
+static void *run_range(void *arg) {
+    Range *r = (Range *)arg;
+    tl_in_parallel = 1;
+    for (size_t i = r->begin; i < r->end; i++) r->fn(r->ctx, i);
+    return NULL;
+}
+
+static void par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), void *ctx) {
+    int threads = g_threads;
+    if (threads > 256) threads = 256;
+    if ((size_t)threads > count) threads = (int)count;
+    if (threads <= 1 || tl_in_parallel || count * work_per_item < (1u << 15)) {
+        for (size_t i = 0; i < count; i++) fn(ctx, i);
+        return;
+    }
+    pthread_t tid[256];
+    Range r[256];
+    for (int t = 0; t < threads; t++) {
+        r[t].fn = fn;
+        r[t].ctx = ctx;
+        r[t].begin = count * (size_t)t / (size_t)threads;
+        r[t].end = count * (size_t)(t + 1) / (size_t)threads;
+        pthread_create(&tid[t], NULL, run_range, &r[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+}
+
+/* rows x cols -> cols x rows, in 16 x 16 blocks; block rows in parallel */
+typedef struct {
+    const uint64_t *src;
+    uint64_t *dst;
+    size_t rows, cols;
+} TrCtx;
+
+static void transpose_rows(void *c, size_t blk) {
+    const TrCtx *x = (const TrCtx *)c;
+    const size_t B = 16, i0 = blk * B;
+    size_t i1 = i0 + B < x->rows ? i0 + B : x->rows;
+    for (size_t j0 = 0; j0 < x->cols; j0 += B) {
+        size_t j1 = j0 + B < x->cols ? j0 + B : x->cols;
+        for (size_t i = i0; i < i1; i++)
+            for (size_t j = j0; j < j1; j++) x->dst[j * x->rows + i] = x->src[i * x->cols + j];
+    }
+}
+
+static void transpose(const uint64_t *src, uint64_t *dst, size_t rows, size_t cols) {
+    TrCtx c = {src, dst, rows, cols};
+    par_for((rows + 15) / 16, 16 * cols, transpose_rows, &c);
+}
+
+typedef struct {
+    const Plan *p;
+    uint64_t *v, *t;
+} CtCtx;
+
+static void ct_cols(void *c, size_t j) {
+    CtCtx *x = (CtCtx *)c;
+    exec(x->p->inner_a, x->t + j * x->p->a);
+}
+static void ct_rows(void *c, size_t i) {
+    CtCtx *x = (CtCtx *)c;
+    exec(x->p->inner_b, x->v + i * x->p->b);
+}
+static void ct_twiddle_row(void *c, size_t j) {
+    CtCtx *x = (CtCtx *)c;
+    const Plan *p = x->p;
+    size_t a = p->a, n = p->n;
+    if (j == 0) return;
+    if (p->shift_twiddles) {
+        size_t step = 384 / n;
+        for (size_t i = 1; i < a; i++) x->t[j * a + i] = or_root_384(x->t[j * a + i], (step * ((i * j) % n)) % 384);
+    } else if (p->twiddles) {
+        for (size_t i = 1; i < a; i++) x->t[j * a + i] = or_mul(x->t[j * a + i], p->twiddles[j * a + i]);
+    } else {
+        /* row j multiplies column i by w^(i*j) */
+        uint64_t wcol = or_pow(p->root, j), wrow = wcol;
+        for (size_t i = 1; i < a; i++) {
+            x->t[j * a + i] = or_mul(x->t[j * a + i], wrow);
+            wrow = or_mul(wrow, wcol);
+        }
+    }
 }
 
 /* cooley_tukey.rs:80-148 (six-step: T(a,b), b x DFT_a, twiddle, T(b,a),
@@ -229,29 +313,12 @@ static void exec_ct(const Plan *p, uint64_t *v) {
     size_t a = p->a, b = p->b, n = p->n;
     uint64_t stack[1024];
     uint64_t *t = scratch(n, stack, 1024);
+    CtCtx cx = {p, v, t};
     transpose(v, t, a, b);
-    for (size_t j = 0; j < b; j++) exec(p->inner_a, t + j * a);
-    if (p->shift_twiddles) {
-        size_t step = 384 / n;
-        for (size_t j = 1; j < b; j++)
-            for (size_t i = 1; i < a; i++)
-                t[j * a + i] = or_root_384(t[j * a + i], (step * ((i * j) % n)) % 384);
-    } else if (p->twiddles) {
-        for (size_t j = 1; j < b; j++)
-            for (size_t i = 1; i < a; i++) t[j * a + i] = or_mul(t[j * a + i], p->twiddles[j * a + i]);
-    } else {
-        uint64_t wcol = p->root;  /* row 1 */
-        for (size_t j = 1; j < b; j++) {
-            uint64_t wrow = wcol;
-            for (size_t i = 1; i < a; i++) {
-                t[j * a + i] = or_mul(t[j * a + i], wrow);
-                wrow = or_mul(wrow, wcol);
-            }
-            wcol = or_mul(wcol, p->root);
-        }
-    }
+    par_for(b, a, ct_cols, &cx);
+    par_for(b, a, ct_twiddle_row, &cx);
     transpose(t, v, b, a);
-    for (size_t i = 0; i < a; i++) exec(p->inner_b, v + i * b);
+    par_for(a, b, ct_rows, &cx);
     transpose(v, t, a, b);
     memcpy(v, t, n * sizeof(uint64_t));
     release(t, stack);

@@ -56,3 +56,41 @@ def test_mixed_device_alone_equals_in_mix(gl, oracle):
         assert torch.equal(a, b)
     for i in (0, 1, len(sizes) - 1):
         assert np.array_equal(from_torch(batched[i]), oracle.ntt(from_torch(xs[i])))
+
+
+@pytest.mark.parametrize("chunk_bytes", [1 << 16, 1 << 20])
+def test_chunked_batches_and_captured_plan(gl, oracle, chunk_bytes):
+    """Small chunks force many grouped chunks plus stand-alone transforms
+    (larger than a chunk); both the eager path and a captured batch plan
+    (replayed several times, with eager work and syncs between replays)
+    match the oracle."""
+    import torch
+    sizes = mixed_sizes(48, 4, 15, 3 + chunk_bytes)
+    prev = gl.set_batch_chunk_bytes(chunk_bytes)
+    try:
+        note(f"chunked batch chunk={chunk_bytes} {len(sizes)} transforms max {max(sizes)}")
+        hs = [oracle.fill_random(n, 500 + i) for i, n in enumerate(sizes)]
+        want = [oracle.ntt(h) for h in hs]
+        ts = [to_torch(h) for h in hs]
+        gl.ntt_batch_mixed(ts)
+        torch.cuda.synchronize()
+        for t, w, n in zip(ts, want, sizes):
+            assert np.array_equal(from_torch(t), w), f"eager n={n}"
+        note(f"captured batch plan chunk={chunk_bytes}")
+        bufs = [to_torch(h) for h in hs]
+        plan = gl.BatchPlan(bufs)
+        try:
+            for rep in range(3):
+                for b, h in zip(bufs, hs):
+                    b.copy_(to_torch(h))
+                y = to_torch(hs[0])
+                gl.ntt(y)  # eager work between replays
+                torch.cuda.synchronize()
+                plan.launch()
+                torch.cuda.synchronize()
+                for b, w, n in zip(bufs, want, sizes):
+                    assert np.array_equal(from_torch(b), w), f"replay {rep} n={n}"
+        finally:
+            plan.close()
+    finally:
+        gl.set_batch_chunk_bytes(prev)
