@@ -71,15 +71,18 @@ def tune(n, verbose=True, max_candidates=400):
     x = torch.empty(n, dtype=torch.int64, device="cuda")
     gl.fill_random(x, 12345)
     gl.plan_set(n, [])
-    base_ms, ref = time_plan(torch, x, n, None)
+    reps = 20 if n <= (1 << 22) else 5
+    base_ms, ref = time_plan(torch, x, n, None, reps)
     default = [p["kernel"] for p in gl.plan_info(n)]
     results = [(base_ms, None)]
     cands = candidate_plans(n)[:max_candidates]
-    for ids in cands:
-        ms, z = time_plan(torch, x, n, ids)
+    for c, ids in enumerate(cands):
+        ms, z = time_plan(torch, x, n, ids, reps)
         if not torch.equal(z, ref):
             raise RuntimeError(f"schedule {ids} for n={n} is not bit-exact")
         results.append((ms, ids))
+        if verbose and (c + 1) % 100 == 0:
+            print(f"  n={n}: {c + 1}/{len(cands)} candidates timed", flush=True)
     gl.plan_set(n, [])
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]

@@ -5,80 +5,133 @@
 // (TW = DFT_256(w_257^(3^q) / 256), precomputed), C[0] += x0, D =
 // DFT_256(C reversed on [1..]), X[3^q mod 257] = D[q].  Generators
 // (gi, gk) = (3, 86) as rader.rs:21-29.  Both DFT_256 run as 16 x 16 with
-// register codelets and two LDS buffers; T = 16 instances per workgroup, one
+// register codelets and one LDS tile; T = 16 instances per workgroup, one
 // DFT_16 task per thread per sub-step.  The pass then applies the usual
 // output twiddle w_M^(m*k) and Stockham store (see PassArgs).
 //
-// a.tw_inner holds four 256-entry tables: w_256^e, TW, 86^q mod 257 and
-// 3^q mod 257.
+// a.tw_inner holds w_256^(k1*j) at j*16 + k1 and the spectrum TW at
+// 256 + k (Montgomery form).  The Rader permutations 86^q and 3^q mod 257
+// are computed in registers: 86^(16*q1 + j) = 86^(16*q1) * 86^j, the first
+// factor a compile-time constant, reduced with 2^8 = -1 (mod 257).
 #include "gl_dft.hpp"
 #include "gl_internal.hpp"
 
 namespace gl {
 
-constexpr int RD_T = 16, RD_NT = 256, RD_STRIDE = 257;
+constexpr int RD_T = 16, RD_NT = 256;
+// One LDS tile: instance t owns RD_TS words, row r of it RD_RS words (odd
+// strides: row-wise and column-wise accesses both spread over the banks).
+constexpr int RD_RS = 17, RD_TS = 16 * RD_RS + 1;
 
-__device__ __forceinline__ int rd_col(int hi, int lo) { return lo * 16 + (hi ^ lo); }
+// x mod 257 for x <= 256 * 256 (x = 256 a + b = b - a, |b - a| <= 256)
+__device__ __forceinline__ u32 mod257(u32 x) {
+    const int r = (int)(x & 255u) - (int)(x >> 8);
+    return (u32)(r < 0 ? r + 257 : r);
+}
 
-__device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *bufA, u64 *bufB) {
+constexpr u32 cx_pow257(u32 g, int e) {
+    u32 r = 1;
+    for (int i = 0; i < e; i++) r = r * g % 257u;
+    return r;
+}
+
+// G^(16 q) mod 257 for q < 16, evaluated at compile time (namespace-scope
+// constexpr tables, indexed by unrolled loop counters).
+struct Pow257x16 {
+    u32 v[16];
+};
+template <u32 G>
+constexpr Pow257x16 pow257_x16() {
+    Pow257x16 t{};
+    for (int q = 0; q < 16; q++) t.v[q] = cx_pow257(G, 16 * q);
+    return t;
+}
+constexpr Pow257x16 kGk16 = pow257_x16<86>(), kGi16 = pow257_x16<3>();
+
+// G^j mod 257 for a runtime j < 16 (square-and-multiply over j's 4 bits)
+template <u32 G>
+__device__ __forceinline__ u32 pow257_small(u32 j) {
+    constexpr u32 g2 = cx_pow257(G, 2), g4 = cx_pow257(G, 4), g8 = cx_pow257(G, 8);
+    u32 r = 1;
+    r = (j & 1u) ? mod257(r * G) : r;
+    r = (j & 2u) ? mod257(r * g2) : r;
+    r = (j & 4u) ? mod257(r * g4) : r;
+    r = (j & 8u) ? mod257(r * g8) : r;
+    return r;
+}
+
+__device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *lds) {
     const int tid = threadIdx.x;
     const int t = tid % RD_T, j = tid / RD_T;  // j in [0, 16)
     const u32 I = blk * (u32)RD_T + (u32)t;
     const bool valid = I < a.total_inst;
-    const u64 *w256 = a.tw_inner, *spec = a.tw_inner + 256, *permk = a.tw_inner + 512,
-              *permi = a.tw_inner + 768;
+    const u64 *w256 = a.tw_inner + j * 16, *spec = a.tw_inner + 256 + j;
     u32 b = 0, i = 0, s = 0, m = 0;
     if (valid) locate(a, I, b, i, s, m);
     const u64 *src = a.in + (u64)b * a.bstride;
-    auto load = [&](u64 r) -> u64 {
-        u64 pos = (u64)m + (u64)a.Mp * r;
-        if (a.flags & FLAG_REVERSE) pos = pos ? a.n_rev - pos : 0;
-        return src[s + a.S * pos];
+    auto load = [&](u32 r) -> u64 {
+        if (a.flags & FLAG_REVERSE) {
+            u64 pos = (u64)m + (u64)a.Mp * r;
+            pos = pos ? a.n_rev - pos : 0;
+            return src[s + a.S * pos];
+        }
+        return src[(u64)i + a.SMp * r];
     };
-    u64 *rowA = bufA + t * RD_STRIDE, *rowB = bufB + t * RD_STRIDE;
-    // sub-step a: DFT_16 over q1 of a[q1*16 + j]
-    u64 x[16];
-    u64 x0 = 0;
+    u64 *row = lds + t * RD_TS;
+    u64 w[16], x[16];
+    // sub-step a: DFT_16 over q1 of a[q1*16 + j] = x[86^(q1*16 + j)]
 #pragma unroll
-    for (int q1 = 0; q1 < 16; q1++) x[q1] = valid ? load(permk[q1 * 16 + j]) : 0;
-    if (j == 0 && valid) x0 = load(0);
+    for (int k1 = 1; k1 < 16; k1++) w[k1] = w256[k1];
+    const u32 gk = pow257_small<86>((u32)j);
+#pragma unroll
+    for (int q1 = 0; q1 < 16; q1++) x[q1] = valid ? load(mod257(kGk16.v[q1] * gk)) : 0;
+    const u64 x0 = (j == 0 && valid) ? load(0) : 0;
     dft<16>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w256[(k1 * j) & 255]);
+    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w[k1]);
 #pragma unroll
-    for (int k1 = 0; k1 < 16; k1++) rowA[k1 * 16 + (j ^ k1)] = x[k1];
+    for (int k1 = 0; k1 < 16; k1++) row[k1 * RD_RS + j] = x[k1];
     __syncthreads();
-    // sub-step b: thread j = k1 holds A[k1 + 16 k2]; pointwise with the
-    // spectrum, X0 = x0 + A[0], C[0] += x0, then store C reversed on [1..].
+    // sub-step b: thread j holds A[j + 16 k2]; pointwise with the spectrum,
+    // X0 = x0 + A[0], C[0] += x0, then C is stored reversed on [1..]:
+    // element p = (256 - k) mod 256 goes to row p mod 16, column p / 16.
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) w[k2] = spec[16 * k2];
+#pragma unroll
+    for (int q2 = 0; q2 < 16; q2++) x[q2] = row[j * RD_RS + q2];
+    dft<16>(x);
     u64 X0 = 0;
 #pragma unroll
-    for (int q2 = 0; q2 < 16; q2++) x[q2] = rowA[j * 16 + (q2 ^ j)];
-    dft<16>(x);
-#pragma unroll
     for (int k2 = 0; k2 < 16; k2++) {
-        const int k = j + 16 * k2;
-        u64 c = mont_mul(x[k2], spec[k]);
-        if (k == 0) {
+        u64 c = mont_mul(x[k2], w[k2]);
+        if (k2 == 0 && j == 0) {
             X0 = add(x0, x[0]);
             c = add(c, x0);
         }
-        const int pos = (256 - k) & 255;
-        rowB[rd_col(pos >> 4, pos & 15)] = c;
+        x[k2] = c;
+    }
+    __syncthreads();  // every sub-step b read is done before the tile is reused
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+        const u32 p = (256u - (u32)(j + 16 * k2)) & 255u;
+        row[(p & 15u) * RD_RS + (p >> 4)] = x[k2];
     }
     __syncthreads();
-    // sub-step c: DFT_16 over q1 of D_in[q1*16 + j]
+    // sub-step c: DFT_16 over q1 of D_in[q1*16 + j] (row j of the tile)
 #pragma unroll
-    for (int q1 = 0; q1 < 16; q1++) x[q1] = rowB[rd_col(q1, j)];
+    for (int k1 = 1; k1 < 16; k1++) w[k1] = w256[k1];
+#pragma unroll
+    for (int q1 = 0; q1 < 16; q1++) x[q1] = row[j * RD_RS + q1];
     dft<16>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w256[(k1 * j) & 255]);
-    __syncthreads();  // rowA is reused: every sub-step b read has completed
-#pragma unroll
-    for (int k1 = 0; k1 < 16; k1++) rowA[k1 * 16 + (j ^ k1)] = x[k1];
+    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w[k1]);
     __syncthreads();
-    // sub-step d: D[j + 16 k2] -> X[3^k mod 257]
 #pragma unroll
-    for (int q2 = 0; q2 < 16; q2++) x[q2] = rowA[j * 16 + (q2 ^ j)];
+    for (int k1 = 0; k1 < 16; k1++) row[k1 * RD_RS + j] = x[k1];
+    __syncthreads();
+    // sub-step d: D[j + 16 k2] -> X[3^(j + 16 k2) mod 257]
+#pragma unroll
+    for (int q2 = 0; q2 < 16; q2++) x[q2] = row[j * RD_RS + q2];
     dft<16>(x);
     if (!valid) return;
     u64 *dst = a.out + (u64)b * a.bstride + s + a.S * 257ull * (u64)m;
@@ -92,19 +145,20 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *b
         if (a.flags & FLAG_SCALE) v = mont_mul(v, a.scale);
         dst[a.S * kk] = v;
     };
+    const u32 hi = pow257_small<3>((u32)j);
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) store(permi[j + 16 * k2], x[k2]);
+    for (int k2 = 0; k2 < 16; k2++) store(mod257(kGi16.v[k2] * hi), x[k2]);
     if (j == 0) store(0, X0);
 }
 
 __global__ __launch_bounds__(RD_NT) void rader257_kernel(PassArgs a) {
-    __shared__ u64 bufA[RD_T * RD_STRIDE], bufB[RD_T * RD_STRIDE];
-    rader257_body(a, blockIdx.x, bufA, bufB);
+    __shared__ u64 lds[RD_T * RD_TS];
+    rader257_body(a, blockIdx.x, lds);
 }
 
 __global__ __launch_bounds__(RD_NT) void rader257_grouped_kernel(const PassArgs *__restrict__ descs,
                                                                  const u32 *__restrict__ starts, u32 nd) {
-    __shared__ u64 bufA[RD_T * RD_STRIDE], bufB[RD_T * RD_STRIDE];
+    __shared__ u64 lds[RD_T * RD_TS];
     const u32 bb = blockIdx.x;
     u32 lo = 0, hi = nd;
     while (hi - lo > 1) {
@@ -112,7 +166,7 @@ __global__ __launch_bounds__(RD_NT) void rader257_grouped_kernel(const PassArgs 
         if (starts[mid] <= bb) lo = mid; else hi = mid;
     }
     const PassArgs a = descs[lo];
-    rader257_body(a, bb - starts[lo], bufA, bufB);
+    rader257_body(a, bb - starts[lo], lds);
 }
 
 static void launch_rader257(const PassArgs &a, unsigned grid, hipStream_t s) {
@@ -124,7 +178,7 @@ static void launch_rader257_grouped(const PassArgs *descs, const u32 *starts, u3
     hipLaunchKernelGGL(rader257_grouped_kernel, dim3(grid), dim3(RD_NT), 0, s, descs, starts, nd);
 }
 
-static const PassVariant kRader257 = {257, 1, RD_T, RD_NT, 2 * RD_T * RD_STRIDE * 8, &launch_rader257,
+static const PassVariant kRader257 = {257, 1, RD_T, RD_NT, RD_T * RD_TS * 8, &launch_rader257,
                                       &launch_rader257_grouped, VK_RADER257, 1};
 
 const PassVariant *rader257_variant() { return &kRader257; }

@@ -299,18 +299,21 @@ const u64 *inner_matrix(u64 R1, u64 R2) {
     return d;
 }
 
-// Tables for the radix-257 Rader pass: w_256^e, the convolution spectrum
-// DFT_256(w_257^(3^q) / 256), 86^q mod 257 and 3^q mod 257 (rader.rs:19-57).
+// Tables for the radix-257 Rader pass (rader.rs:19-57), Montgomery form:
+// [0, 256) w_256^(k1*j) at entry j*16 + k1 (a thread's inner twiddles are
+// consecutive), [256, 512) the convolution spectrum DFT_256(w_257^(3^q)/256).
 const u64 *rader257_table() {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     if (g_rader257) return g_rader257;
-    std::vector<u64> t(1024);
+    std::vector<u64> pw(256), t(512);
     u64 w256 = root(256), w257 = root(257), s256 = inv(256);
     u64 v = 1;
     for (int e = 0; e < 256; e++) {
-        t[e] = v;
+        pw[e] = v;
         v = mul(v, w256);
     }
+    for (int j = 0; j < 16; j++)
+        for (int k1 = 0; k1 < 16; k1++) t[j * 16 + k1] = pw[(k1 * j) & 255];
     std::vector<u64> c(256);
     u64 gq = 1;
     for (int q = 0; q < 256; q++) {
@@ -319,17 +322,10 @@ const u64 *rader257_table() {
     }
     for (int k = 0; k < 256; k++) {
         u64 acc = 0;
-        for (int q = 0; q < 256; q++) acc = add(acc, mul(c[q], t[(q * k) & 255]));
+        for (int q = 0; q < 256; q++) acc = add(acc, mul(c[q], pw[(q * k) & 255]));
         t[256 + k] = acc;
     }
     for (int e = 0; e < 512; e++) t[e] = to_mont(t[e]);
-    u64 pk = 1, pi = 1;
-    for (int q = 0; q < 256; q++) {
-        t[512 + q] = pk;
-        t[768 + q] = pi;
-        pk = pk * 86 % 257;
-        pi = pi * 3 % 257;
-    }
     g_rader257 = upload(t);
     return g_rader257;
 }
@@ -491,7 +487,13 @@ static bool build_plan(u64 n, Plan &plan) {
     }
     double best_score = 1e30;
     std::vector<std::pair<int, int>> best;  // (bits, g)
-    for (int P = 1; P <= 4 && best.empty(); P++) {
+    // Passes of at most ~2^8 points (two register stages) run most
+    // efficiently; start from that pass count, at most three passes, and
+    // add passes only when no split fits the compiled variants.
+    double lg = (double)k;
+    for (int q : primes) lg += std::log2((double)q);
+    const int p_start = std::max(1, std::min(3, (int)std::ceil(lg / 8.0)));
+    for (int P = p_start; P <= 4 && best.empty(); P++) {
         for (auto &grp : groupings) {
             if ((int)grp.size() > P) continue;
             std::vector<int> g(P, 1), bits(P, 0), cap(P, 12), lo(P, 1);
