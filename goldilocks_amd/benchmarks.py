@@ -10,10 +10,12 @@ def mixed_sizes(count, seed, lo=10, hi=20, factors=(1, 3, 5, 15, 17, 257)):
 
 
 def bench_batch(args, torch, gl):
-    """1024 forward transforms, n = 2^k * f (k uniform in [10, 20], f from
-    {1, 3, 5, 15, 17, 257}), device-resident: grouped launches vs one launch
-    sequence per transform; per-batch latency p50/p99 over the repetitions."""
-    sizes = mixed_sizes(1024, 7)
+    """1024 forward transforms, n = 2^k * f (k uniform in [lo, hi], default
+    [10, 20], f from {1, 3, 5, 15, 17, 257}), device-resident: grouped
+    launches vs one launch sequence per transform; per-batch latency p50/p99
+    over the repetitions."""
+    lo, hi = args.batch_k
+    sizes = mixed_sizes(1024, 7, lo, hi)
     total = sum(sizes)
     bufs = [torch.empty(n, dtype=torch.int64, device="cuda") for n in sizes]
     for i, b in enumerate(bufs):
@@ -51,7 +53,7 @@ def bench_batch(args, torch, gl):
         "higher_is_better": True,
         "dtype": "u64",
         "data": "synthetic: seeded uniform [0,p) words, device-resident",
-        "config": {"workload": "1024 forward NTTs, n = 2^k * f, k uniform in [10,20], f in {1,3,5,15,17,257}",
+        "config": {"workload": f"1024 forward NTTs, n = 2^k * f, k uniform in [{lo},{hi}], f in {{1,3,5,15,17,257}}",
                    "elements": total, "bytes": 8 * total},
         "latency_ms": {"p50": res["batched"]["ms_p50"], "p99": res["batched"]["ms_p99"]},
         "batched": res["batched"],

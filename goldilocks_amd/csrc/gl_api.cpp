@@ -38,7 +38,7 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
     if (rc) return rc;
     if (inverse && n <= 1) return GL_OK;  // mod.rs:105-107
     const Plan *plan = nullptr;
-    if ((rc = get_plan(n, &plan))) return rc;
+    if ((rc = S0 > 1 ? get_plan_cols(n, &plan) : get_plan(n, &plan))) return rc;
     if (count == 0 || S0 == 0 || plan->passes.empty()) return GL_OK;
     if (!d) return set_error(GL_E_ARG, "null device pointer");
     const u64 bstride = n * S0;
@@ -48,6 +48,10 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
     return run_plan(*plan, d, tmp, S0, count, bstride, inverse, s);
 }
 
+// 21 words of per-stream device scratch for a layer's padded partial block
+// (layers on one stream run in order, so they share it).
+u64 *k12_stage(hipStream_t s) { return stream_scratch(s, 32, 6); }
+
 int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
                     hipStream_t s) {
     size_t need = tree_size(len);
@@ -55,8 +59,10 @@ int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
     const uint8_t *layer = d_in;
     uint8_t *dst = d_out;
+    u64 *stage = k12_stage(s);
+    if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
     while (len > HASH) {
-        launch_k12_layer(layer, len, dst, s);
+        launch_k12_layer(layer, len, dst, stage, s);
         GL_HIP(hipGetLastError());
         layer = dst;
         len = next_layer(len);
@@ -88,10 +94,11 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     const u64 ext = (u64)rows * blowup;
     const Plan *prow = nullptr, *pext = nullptr;
     int rc;
-    if ((rc = get_plan(rows, &prow))) return rc;
-    if ((rc = get_plan(ext, &pext))) return rc;
-    GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
-    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s))) return rc;
+    if ((rc = cols > 1 ? get_plan_cols(rows, &prow) : get_plan(rows, &prow))) return rc;
+    if ((rc = cols > 1 ? get_plan_cols(ext, &pext) : get_plan(ext, &pext))) return rc;
+    // intt of the columns, reading the input matrix in the first pass
+    if (prow->passes.empty()) GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
+    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m))) return rc;
     PowTable g = power_table(2717, rows);
     if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
@@ -272,7 +279,9 @@ int gl_k12_layer_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, void *stre
     int rc = ensure_init();
     if (rc) return rc;
     if (len && (!d_in || !d_out)) return set_error(GL_E_ARG, "null device pointer");
-    launch_k12_layer(d_in, len, d_out, as_stream(stream));
+    u64 *stage = k12_stage(as_stream(stream));
+    if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    launch_k12_layer(d_in, len, d_out, stage, as_stream(stream));
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -288,7 +297,9 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + ((len + 255) / 256 * 256);
     GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
-    launch_k12_layer(din, len, dout, s);
+    u64 *stage = k12_stage(s);
+    if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    launch_k12_layer(din, len, dout, stage, s);
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out, dout, olen, hipMemcpyDeviceToHost, s));
     GL_HIP(hipStreamSynchronize(s));

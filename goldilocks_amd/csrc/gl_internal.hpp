@@ -112,7 +112,8 @@ void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blow
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 
 // K12 kernels (gl_k12.hip).
-void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, hipStream_t s);
+// stage: 21 words of device scratch (the padded partial last block)
+void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage, hipStream_t s);
 void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hipStream_t s);
 void launch_pow_search(const uint8_t *challenge32, u64 threshold, u64 start, u64 count,
                        unsigned long long *best, hipStream_t s);

@@ -11,79 +11,113 @@
 
 namespace gl {
 
-__device__ __constant__ u64 kRC[12] = {
+constexpr u64 kRC[12] = {
     0x000000008000808bull, 0x800000000000008bull, 0x8000000000008089ull,
     0x8000000000008003ull, 0x8000000000008002ull, 0x8000000000000080ull,
     0x000000000000800aull, 0x800000008000000aull, 0x8000000080008081ull,
     0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull,
 };
 
-__device__ __forceinline__ u64 rotl64(u64 x, int r) { return (x << r) | (x >> (64 - r)); }
+// A 64-bit lane as two 32-bit halves.
+struct Lane {
+    u32 lo, hi;
+};
 
-// 12 rounds of theta, rho+pi, chi, iota on lanes a[x + 5y].
-__device__ __forceinline__ void keccak_p12(u64 *a) {
-#pragma unroll 1
-    for (int round = 0; round < 12; round++) {
-        u64 c0 = a[0] ^ a[5] ^ a[10] ^ a[15] ^ a[20];
-        u64 c1 = a[1] ^ a[6] ^ a[11] ^ a[16] ^ a[21];
-        u64 c2 = a[2] ^ a[7] ^ a[12] ^ a[17] ^ a[22];
-        u64 c3 = a[3] ^ a[8] ^ a[13] ^ a[18] ^ a[23];
-        u64 c4 = a[4] ^ a[9] ^ a[14] ^ a[19] ^ a[24];
-        u64 d0 = c4 ^ rotl64(c1, 1), d1 = c0 ^ rotl64(c2, 1), d2 = c1 ^ rotl64(c3, 1);
-        u64 d3 = c2 ^ rotl64(c4, 1), d4 = c3 ^ rotl64(c0, 1);
-        // theta + rho + pi: b[y, 2x+3y] = rot(a[x, y] ^ d[x], r[x, y])
-        u64 b00 = a[0] ^ d0;
-        u64 b10 = rotl64(a[6] ^ d1, 44);
-        u64 b20 = rotl64(a[12] ^ d2, 43);
-        u64 b30 = rotl64(a[18] ^ d3, 21);
-        u64 b40 = rotl64(a[24] ^ d4, 14);
-        u64 b01 = rotl64(a[3] ^ d3, 28);
-        u64 b11 = rotl64(a[9] ^ d4, 20);
-        u64 b21 = rotl64(a[10] ^ d0, 3);
-        u64 b31 = rotl64(a[16] ^ d1, 45);
-        u64 b41 = rotl64(a[22] ^ d2, 61);
-        u64 b02 = rotl64(a[1] ^ d1, 1);
-        u64 b12 = rotl64(a[7] ^ d2, 6);
-        u64 b22 = rotl64(a[13] ^ d3, 25);
-        u64 b32 = rotl64(a[19] ^ d4, 8);
-        u64 b42 = rotl64(a[20] ^ d0, 18);
-        u64 b03 = rotl64(a[4] ^ d4, 27);
-        u64 b13 = rotl64(a[5] ^ d0, 36);
-        u64 b23 = rotl64(a[11] ^ d1, 10);
-        u64 b33 = rotl64(a[17] ^ d2, 15);
-        u64 b43 = rotl64(a[23] ^ d3, 56);
-        u64 b04 = rotl64(a[2] ^ d2, 62);
-        u64 b14 = rotl64(a[8] ^ d3, 55);
-        u64 b24 = rotl64(a[14] ^ d4, 39);
-        u64 b34 = rotl64(a[15] ^ d0, 41);
-        u64 b44 = rotl64(a[21] ^ d1, 2);
-        // chi (row y holds b0y..b4y as x = 0..4) + iota
-        a[0] = b00 ^ (~b10 & b20) ^ kRC[round];
-        a[1] = b10 ^ (~b20 & b30);
-        a[2] = b20 ^ (~b30 & b40);
-        a[3] = b30 ^ (~b40 & b00);
-        a[4] = b40 ^ (~b00 & b10);
-        a[5] = b01 ^ (~b11 & b21);
-        a[6] = b11 ^ (~b21 & b31);
-        a[7] = b21 ^ (~b31 & b41);
-        a[8] = b31 ^ (~b41 & b01);
-        a[9] = b41 ^ (~b01 & b11);
-        a[10] = b02 ^ (~b12 & b22);
-        a[11] = b12 ^ (~b22 & b32);
-        a[12] = b22 ^ (~b32 & b42);
-        a[13] = b32 ^ (~b42 & b02);
-        a[14] = b42 ^ (~b02 & b12);
-        a[15] = b03 ^ (~b13 & b23);
-        a[16] = b13 ^ (~b23 & b33);
-        a[17] = b23 ^ (~b33 & b43);
-        a[18] = b33 ^ (~b43 & b03);
-        a[19] = b43 ^ (~b03 & b13);
-        a[20] = b04 ^ (~b14 & b24);
-        a[21] = b14 ^ (~b24 & b34);
-        a[22] = b24 ^ (~b34 & b44);
-        a[23] = b34 ^ (~b44 & b04);
-        a[24] = b44 ^ (~b04 & b14);
+__device__ __forceinline__ u32 xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+
+// rotl64 by a compile-time R in (0, 64): two funnel shifts (v_alignbit_b32).
+template <int R>
+__device__ __forceinline__ Lane rotl(Lane x) {
+    if constexpr (R == 0) {
+        return x;
+    } else if constexpr (R == 32) {
+        return {x.hi, x.lo};
+    } else if constexpr (R > 32) {
+        return rotl<R - 32>(Lane{x.hi, x.lo});
+    } else {
+        return {__builtin_amdgcn_alignbit(x.lo, x.hi, 32 - R), __builtin_amdgcn_alignbit(x.hi, x.lo, 32 - R)};
     }
+}
+
+// chi: b0 ^ (~b1 & b2) on both halves
+__device__ __forceinline__ Lane chi(Lane b0, Lane b1, Lane b2) {
+    return {b0.lo ^ (~b1.lo & b2.lo), b0.hi ^ (~b1.hi & b2.hi)};
+}
+
+// theta's column sums and rho: b = rotl(a ^ C[x-1] ^ rotl1(C[x+1]), r)
+template <int R>
+__device__ __forceinline__ Lane theta_rho(Lane a, Lane cm, Lane cp1) {
+    return rotl<R>(Lane{xor3(a.lo, cm.lo, cp1.lo), xor3(a.hi, cm.hi, cp1.hi)});
+}
+
+// One round on lanes a[x + 5y] (theta, rho + pi, chi, iota).
+template <int ROUND>
+__device__ __forceinline__ void keccak_round(Lane *a) {
+    Lane c[5], r[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        c[x].lo = xor3(xor3(a[x].lo, a[x + 5].lo, a[x + 10].lo), a[x + 15].lo, a[x + 20].lo);
+        c[x].hi = xor3(xor3(a[x].hi, a[x + 5].hi, a[x + 10].hi), a[x + 15].hi, a[x + 20].hi);
+    }
+#pragma unroll
+    for (int x = 0; x < 5; x++) r[x] = rotl<1>(c[x]);
+    // b[y, 2x+3y] = rot(a[x, y] ^ d[x], rho[x, y]), d[x] = c[x-1] ^ rotl1(c[x+1])
+#define TR(X, Y, RHO) theta_rho<RHO>(a[(X) + 5 * (Y)], c[((X) + 4) % 5], r[((X) + 1) % 5])
+    const Lane b00 = TR(0, 0, 0), b10 = TR(1, 1, 44), b20 = TR(2, 2, 43), b30 = TR(3, 3, 21), b40 = TR(4, 4, 14);
+    const Lane b01 = TR(3, 0, 28), b11 = TR(4, 1, 20), b21 = TR(0, 2, 3), b31 = TR(1, 3, 45), b41 = TR(2, 4, 61);
+    const Lane b02 = TR(1, 0, 1), b12 = TR(2, 1, 6), b22 = TR(3, 2, 25), b32 = TR(4, 3, 8), b42 = TR(0, 4, 18);
+    const Lane b03 = TR(4, 0, 27), b13 = TR(0, 1, 36), b23 = TR(1, 2, 10), b33 = TR(2, 3, 15), b43 = TR(3, 4, 56);
+    const Lane b04 = TR(2, 0, 62), b14 = TR(3, 1, 55), b24 = TR(4, 2, 39), b34 = TR(0, 3, 41), b44 = TR(1, 4, 2);
+#undef TR
+    // chi (row y holds b0y..b4y as x = 0..4) + iota
+    constexpr u32 rc_lo = (u32)kRC[ROUND], rc_hi = (u32)(kRC[ROUND] >> 32);
+    a[0] = chi(b00, b10, b20);
+    a[0].lo ^= rc_lo;
+    a[0].hi ^= rc_hi;
+    a[1] = chi(b10, b20, b30);
+    a[2] = chi(b20, b30, b40);
+    a[3] = chi(b30, b40, b00);
+    a[4] = chi(b40, b00, b10);
+    a[5] = chi(b01, b11, b21);
+    a[6] = chi(b11, b21, b31);
+    a[7] = chi(b21, b31, b41);
+    a[8] = chi(b31, b41, b01);
+    a[9] = chi(b41, b01, b11);
+    a[10] = chi(b02, b12, b22);
+    a[11] = chi(b12, b22, b32);
+    a[12] = chi(b22, b32, b42);
+    a[13] = chi(b32, b42, b02);
+    a[14] = chi(b42, b02, b12);
+    a[15] = chi(b03, b13, b23);
+    a[16] = chi(b13, b23, b33);
+    a[17] = chi(b23, b33, b43);
+    a[18] = chi(b33, b43, b03);
+    a[19] = chi(b43, b03, b13);
+    a[20] = chi(b04, b14, b24);
+    a[21] = chi(b14, b24, b34);
+    a[22] = chi(b24, b34, b44);
+    a[23] = chi(b34, b44, b04);
+    a[24] = chi(b44, b04, b14);
+}
+
+template <int ROUND>
+__device__ __forceinline__ void keccak_rounds(Lane *a) {
+    if constexpr (ROUND < 12) {
+        keccak_round<ROUND>(a);
+        keccak_rounds<ROUND + 1>(a);
+    }
+}
+
+// Keccak-p[1600, 12] (rounds 12..23 of Keccak-f; generic.rs:25-60) on
+// 64-bit lanes a[x + 5y], fully unrolled so the pi lane map is register
+// renaming.
+__device__ __forceinline__ void keccak_p12(u64 *a) {
+    Lane l[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) l[i] = Lane{(u32)a[i], (u32)(a[i] >> 32)};
+    keccak_rounds<0>(l);
+#pragma unroll
+    for (int i = 0; i < 25; i++) a[i] = ((u64)l[i].hi << 32) | l[i].lo;
 }
 
 // Byte-wise absorb of a block of l <= 166 bytes with K12 padding.  Lane
@@ -105,27 +139,30 @@ __device__ __forceinline__ void absorb_padded(u64 *a, const uint8_t *in, u64 l) 
     a[20] ^= 0x8000000000000000ull;
 }
 
-// out[32i .. 32i+32] = K12(in[160i .. min(160i+160, len)]).
-__global__ __launch_bounds__(256) void k12_layer_kernel(const uint8_t *__restrict__ in, u64 len,
-                                                        uint8_t *__restrict__ out, u64 nblocks,
-                                                        int aligned) {
+// out[32i .. 32i+32] = K12(in[160i .. 160i+160]) for the nfull full blocks
+// of an 8-byte-aligned layer: 20 word loads, the padding words are constants
+// (0x00, 0x07 at bytes 160, 161; 0x80 at byte 167).  Block nfull, when
+// nblocks = nfull + 1, is the partial last block, read from the 21 padded
+// words k12_stage_tail_kernel wrote.  Kept free of byte-wise paths so the
+// state and the round temporaries fit in registers.
+__global__ __launch_bounds__(256) void k12_layer_kernel(const u64 *__restrict__ in, u64 nfull, u64 nblocks,
+                                                        const u64 *__restrict__ stage, u64 *__restrict__ out) {
     for (u64 blk = blockIdx.x * (u64)blockDim.x + threadIdx.x; blk < nblocks;
          blk += (u64)gridDim.x * blockDim.x) {
         u64 a[25];
-        const u64 off = blk * 160;
-        const u64 l = (len - off) < 160 ? (len - off) : 160;
-        if (l == 160 && aligned) {
-            const u64 *w = (const u64 *)(in + off);
+        if (blk < nfull) {
+            const u64 *w = in + blk * 20;
 #pragma unroll
             for (int i = 0; i < 20; i++) a[i] = w[i];
-            a[20] = 0x8000000000000700ull;  // 0x00, 0x07 at bytes 160,161; 0x80 at 167
-#pragma unroll
-            for (int i = 21; i < 25; i++) a[i] = 0;
+            a[20] = 0x8000000000000700ull;
         } else {
-            absorb_padded(a, in + off, l);
+#pragma unroll
+            for (int i = 0; i < 21; i++) a[i] = stage[i];
         }
+#pragma unroll
+        for (int i = 21; i < 25; i++) a[i] = 0;
         keccak_p12(a);
-        u64 *o = (u64 *)(out + blk * 32);
+        u64 *o = out + blk * 4;
         o[0] = a[0];
         o[1] = a[1];
         o[2] = a[2];
@@ -133,13 +170,60 @@ __global__ __launch_bounds__(256) void k12_layer_kernel(const uint8_t *__restric
     }
 }
 
-void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, hipStream_t s) {
-    u64 nblocks = (len + 159) / 160;
+// The partial block [160 first, len) padded into 21 words (lane w builds
+// word w): data bytes, 0x00 at byte l, 0x07 at byte l + 1, 0x80 at byte 167.
+__global__ __launch_bounds__(64) void k12_stage_tail_kernel(const uint8_t *__restrict__ in, u64 len, u64 first,
+                                                            u64 *__restrict__ stage) {
+    const int w = threadIdx.x;
+    if (w >= 21) return;
+    const uint8_t *blk = in + first * 160;
+    const u64 l = len - first * 160;
+    u64 v = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const u64 j = 8 * (u64)w + b;
+        if (j < l) v |= (u64)blk[j] << (8 * b);
+        else if (j == l + 1) v |= (u64)0x07 << (8 * b);
+    }
+    if (w == 20) v ^= 0x8000000000000000ull;
+    stage[w] = v;
+}
+
+// Blocks [first, nblocks) byte by byte: every block of a layer whose input
+// or output is not 8-byte aligned.
+__global__ __launch_bounds__(64) void k12_layer_bytes_kernel(const uint8_t *__restrict__ in, u64 len,
+                                                             uint8_t *__restrict__ out, u64 first, u64 nblocks) {
+    for (u64 blk = first + blockIdx.x * (u64)blockDim.x + threadIdx.x; blk < nblocks;
+         blk += (u64)gridDim.x * blockDim.x) {
+        u64 a[25];
+        const u64 off = blk * 160;
+        const u64 l = (len - off) < 160 ? (len - off) : 160;
+        absorb_padded(a, in + off, l);
+        keccak_p12(a);
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int b = 0; b < 8; b++) out[blk * 32 + 8 * w + b] = (uint8_t)(a[w] >> (8 * b));
+    }
+}
+
+void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage, hipStream_t s) {
+    const u64 nblocks = (len + 159) / 160;
     if (nblocks == 0) return;
+    const bool aligned = ((uintptr_t)in % 8) == 0 && ((uintptr_t)out % 8) == 0;
+    if (!aligned) {  // every block byte by byte
+        u64 g = (nblocks + 63) / 64;
+        unsigned grid = (unsigned)(g < 65536 ? g : 65536);
+        hipLaunchKernelGGL(k12_layer_bytes_kernel, dim3(grid), dim3(64), 0, s, in, len, out, (u64)0, nblocks);
+        return;
+    }
+    const u64 nfull = len / 160;
+    if (nfull < nblocks)
+        hipLaunchKernelGGL(k12_stage_tail_kernel, dim3(1), dim3(64), 0, s, in, len, nfull, stage);
     u64 g = (nblocks + 255) / 256;
     unsigned grid = (unsigned)(g < 65536 ? g : 65536);
-    int aligned = ((uintptr_t)in % 8) == 0;
-    hipLaunchKernelGGL(k12_layer_kernel, dim3(grid), dim3(256), 0, s, in, len, out, nblocks, aligned);
+    hipLaunchKernelGGL(k12_layer_kernel, dim3(grid), dim3(256), 0, s, (const u64 *)in, nfull, nblocks,
+                       (const u64 *)stage, (u64 *)out);
 }
 
 // Proof-of-work scan (pcs/src/pow.rs:32-60): nonces [start, start+count) are

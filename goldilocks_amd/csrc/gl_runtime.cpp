@@ -423,6 +423,7 @@ bool plan_is_composite(const Plan &plan) {
 // ---------------------------------------------------------------- planner
 static std::mutex g_plan_mu;
 static std::map<u64, std::unique_ptr<Plan>> g_plans;
+static std::map<u64, std::unique_ptr<Plan>> g_plans_cols;  // column transforms (S0 > 1)
 
 static const PassVariant *find_variant(int R1, int R2) {
     int cnt = 0;
@@ -433,7 +434,11 @@ static const PassVariant *find_variant(int R1, int R2) {
 }
 
 // Pass with `bits` power-of-two bits and odd codelet g (g = 1: none).
-static const PassVariant *variant_for(int bits, int g) {
+// Column transforms (cols) use two-sub-step variants only: their instances
+// are adjacent columns, and those variants load 16-32 of them per row
+// segment where the three-stage ones load 2-8.
+static const PassVariant *variant_for(int bits, int g, bool cols = false) {
+    if (g == 1 && cols && bits > 8) return nullptr;
     if (g == 1) {
         // Power-of-two passes: two register stages up to 2^8, three stages
         // (16-register codelets, four waves per SIMD) above; chosen from the
@@ -448,7 +453,7 @@ static const PassVariant *variant_for(int bits, int g) {
     return find_variant(1 << bits, g);
 }
 
-static bool build_plan(u64 n, Plan &plan) {
+static bool build_plan(u64 n, Plan &plan, bool cols = false) {
     plan.n = n;
     plan.passes.clear();
     if (n == 1) return true;
@@ -493,7 +498,7 @@ static bool build_plan(u64 n, Plan &plan) {
     double lg = (double)k;
     for (int q : primes) lg += std::log2((double)q);
     const int p_start = std::max(1, std::min(3, (int)std::ceil(lg / 8.0)));
-    for (int P = p_start; P <= 4 && best.empty(); P++) {
+    for (int P = p_start; P <= 5 && best.empty(); P++) {
         for (auto &grp : groupings) {
             if ((int)grp.size() > P) continue;
             std::vector<int> g(P, 1), bits(P, 0), cap(P, 12), lo(P, 1);
@@ -527,7 +532,7 @@ static bool build_plan(u64 n, Plan &plan) {
             bool ok = true;
             double mx = 0;
             for (int i = 0; i < P; i++) {
-                if (!variant_for(bits[i], g[i])) ok = false;
+                if (!variant_for(bits[i], g[i], cols)) ok = false;
                 mx = std::max(mx, bits[i] + std::log2((double)g[i]));
             }
             if (!ok) continue;
@@ -540,7 +545,7 @@ static bool build_plan(u64 n, Plan &plan) {
     }
     if (best.empty()) return false;
     for (auto &bg : best) {
-        const PassVariant *v = variant_for(bg.first, bg.second);
+        const PassVariant *v = variant_for(bg.first, bg.second, cols);
         plan.passes.push_back({v, (u64)v->R1 * (u64)v->R2});
     }
     return true;
@@ -557,6 +562,22 @@ int get_plan(u64 n, const Plan **out) {
         if (!build_plan(n, *p))
             return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
         it = g_plans.emplace(n, std::move(p)).first;
+    }
+    *out = it->second.get();
+    return GL_OK;
+}
+
+int get_plan_cols(u64 n, const Plan **out) {
+    if (n == 0 || (P - 1) % n != 0)
+        return set_error(GL_E_SIZE, std::to_string(n) +
+                                        " is not a supported NTT size (does not divide the multiplicative order)");
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans_cols.find(n);
+    if (it == g_plans_cols.end()) {
+        std::unique_ptr<Plan> p(new Plan());
+        if (!build_plan(n, *p, true))
+            return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
+        it = g_plans_cols.emplace(n, std::move(p)).first;
     }
     *out = it->second.get();
     return GL_OK;
@@ -799,10 +820,11 @@ int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count) {
 }
 
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream, bool scale_out) {
+             hipStream_t stream, bool scale_out, const u64 *input) {
     if (plan.passes.empty() || count == 0) return GL_OK;
     PassCursor c{&plan, data, tmp, S0, count, bstride, inverse};
     c.scale_out = scale_out;
+    c.input = input;
     c.begin();
     while (!c.done()) {
         const PassVariant *v = plan.passes[c.p].v;
@@ -827,6 +849,7 @@ void clear_caches() {
     {
         std::lock_guard<std::mutex> lk(g_plan_mu);
         g_plans.clear();
+        g_plans_cols.clear();
     }
     {
         std::lock_guard<std::mutex> lk(g_tw_mu);

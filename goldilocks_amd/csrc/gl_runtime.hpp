@@ -75,6 +75,8 @@ struct Plan {
 };
 // Returns GL_OK and a pass list for length n, GL_E_SIZE or GL_E_UNSUPPORTED.
 int get_plan(u64 n, const Plan **out);
+// Schedule for column transforms (S0 > 1 interleaved transforms).
+int get_plan_cols(u64 n, const Plan **out);
 // Install an explicit pass list (variant indices as listed by gl_variants;
 // index == variant count means the radix-257 Rader pass); empty restores
 // the planner's default.
@@ -89,11 +91,12 @@ struct PassCursor {
     u64 S0, count, bstride;
     bool inverse;
     bool scale_out = true;  // inverse: multiply the last pass by 1/n
+    const u64 *input = nullptr;  // first pass reads here instead of data (out of place)
     // evolving state
     size_t p = 0;
     u64 S = 0, M = 0;
-    u64 *src = nullptr;
-    void begin() { p = 0; S = S0; M = plan->n; src = data; }
+    const u64 *src = nullptr;
+    void begin() { p = 0; S = S0; M = plan->n; src = input ? input : data; }
     bool done() const { return p >= plan->passes.size(); }
     // Fills args/grid for the current pass and advances; returns GL_OK or an error.
     int next(PassArgs &a, unsigned &grid);
@@ -102,8 +105,10 @@ struct PassCursor {
 // Run `count` items of bstride words, each holding S0 interleaved transforms
 // of length n (S0 = 1: contiguous transforms; S0 = cols: matrix columns).
 // tmp must hold count*bstride words when the plan has more than one pass.
+// input (optional): the first pass reads the transform from there and the
+// result lands in data (same layout), leaving input untouched.
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream, bool scale_out = true);
+             hipStream_t stream, bool scale_out = true, const u64 *input = nullptr);
 // True when the plan contains a pass that is not a single kernel (the
 // radix-65537 Rader pass), which grouped batch launches cannot include.
 bool plan_is_composite(const Plan &plan);

@@ -58,22 +58,39 @@ def kernel_stats(path):
     return sorted(rows, key=lambda x: -x[2])
 
 
+def write_stats_md(f, args, stats):
+    f.write(f"# rocprofv3 summary: {args.workload}\n\n")
+    if args.command:
+        f.write("Commands (repo root, one MI355X):\n\n")
+        for c in args.command.split(";;"):
+            f.write(f"    {c.strip()}\n")
+        f.write("\n")
+    f.write("Kernel time (`--kernel-trace --stats`, kernel_stats.csv):\n\n")
+    f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
+    for name, calls, tot, avg, pct in stats[:12]:
+        f.write(f"| `{short(name)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", required=True)
-    ap.add_argument("--copy-bytes", type=int, required=True)
+    ap.add_argument("--copy-bytes", type=int, default=0)
     ap.add_argument("--stats", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", default=None)
+    ap.add_argument("--write", default=None)
     ap.add_argument("--algorithmic-bytes", type=int, default=None,
                     help="algorithmic bytes per launch of the pass kernels (16 n for an NTT pass)")
-    ap.add_argument("--json", required=True)
+    ap.add_argument("--json", default=None)
     ap.add_argument("--md", required=True)
     ap.add_argument("--command", default="")
     args = ap.parse_args()
+    stats = kernel_stats(args.stats)
+    if not (args.fetch and args.write):
+        with open(args.md, "w") as f:
+            write_stats_md(f, args, stats)
+        return
     fs, fetch = calibrated(args.fetch, "FETCH_SIZE", args.copy_bytes)
     ws, write = calibrated(args.write, "WRITE_SIZE", args.copy_bytes)
-    stats = kernel_stats(args.stats)
     data = {}
     if os.path.exists(args.json):
         with open(args.json) as f:
@@ -92,16 +109,7 @@ def main():
     with open(args.json, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
     with open(args.md, "w") as f:
-        f.write(f"# rocprofv3 summary: {args.workload}\n\n")
-        if args.command:
-            f.write("Commands (repo root, one MI355X):\n\n")
-            for c in args.command.split(";;"):
-                f.write(f"    {c.strip()}\n")
-            f.write("\n")
-        f.write("Kernel time (`--kernel-trace --stats`, kernel_stats.csv):\n\n")
-        f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
-        for name, calls, tot, avg, pct in stats[:12]:
-            f.write(f"| `{short(name)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |\n")
+        write_stats_md(f, args, stats)
         f.write(f"\nHBM traffic per launch (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`; one counter unit = "
                 f"{fs:.1f} B read / {ws:.1f} B written, from the {args.copy_bytes}-byte copy):\n\n")
         f.write("| kernel | launches | fetch MiB | write MiB | total MiB |")
