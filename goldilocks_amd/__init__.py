@@ -22,7 +22,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "P", "ntt", "intt", "strategy", "Strategy", "clear_cache", "is_divisor",
+    "P", "ntt", "intt", "strategy", "Strategy", "clear_cache", "is_divisor", "next_size",
     "merkle_tree", "process_layer", "k12", "next_layer_size", "merkle_tree_size",
     "commit", "ntt_batch_mixed", "fill_random", "lib", "GoldilocksError",
 ]
@@ -63,6 +63,7 @@ def lib():
             "gl_last_error": (ctypes.c_char_p, []),
             "gl_init": (i32, [i32]),
             "gl_is_divisor": (i32, [u64]),
+            "gl_next_size": (u64, [u64]),
             "gl_ntt": (i32, [vp, sz]),
             "gl_intt": (i32, [vp, sz]),
             "gl_ntt_batch": (i32, [vp, sz, sz, i32]),
@@ -147,6 +148,11 @@ def _dev_ptr(t):
 def is_divisor(n):
     """ntt/src/divisors.rs:21-23."""
     return bool(lib().gl_is_divisor(n))
+
+
+def next_size(n):
+    """divisors::next (ntt/src/divisors.rs:52-59): smallest supported length >= n."""
+    return lib().gl_next_size(n)
 
 
 def _transform(values, n, count, inverse):

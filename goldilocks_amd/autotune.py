@@ -92,7 +92,7 @@ def tune(n, verbose=True, max_candidates=400):
         vs = {v["id"]: v for v in gl.variants()}
         def desc(ids):
             return "default " + str(default) if ids is None else \
-                " + ".join(f"{vs[i]['R1']}x{vs[i]['R2']}/T{vs[i]['T']}" for i in ids)
+                " + ".join(tune_key(vs[i]) for i in ids)
         print(f"n={n}: {len(cands)} candidates; default {base_ms * 1e3:.1f} us", flush=True)
         for ms, ids in results[:8]:
             print(f"   {ms * 1e3:9.1f} us  {desc(ids)}", flush=True)
@@ -102,13 +102,18 @@ def tune(n, verbose=True, max_candidates=400):
 TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.txt")
 
 
+def tune_key(v):
+    """Variant as written in the tune file: R1xR2/T."""
+    return f"{v['R1']}x{v['R2']}/{v['T']}"
+
+
 def write_tune_file(entries, path=TUNE_FILE):
     """Write schedules in the text format the library loads at init:
     "n R1xR2/T R1xR2/T ..." per line (merged with existing lines).  R2 is the
     variant's R2 field (R2*R3 for three-stage kernels); (R1, R2, T) names one
     compiled variant."""
     vs = {v["id"]: v for v in gl.variants()}
-    keys = [(v["R1"], v["R2"], v["T"]) for v in vs.values()]
+    keys = [tune_key(v) for v in vs.values()]
     assert len(keys) == len(set(keys)), "tune-file keys must name one variant each"
     lines = {}
     if os.path.exists(path):
@@ -117,7 +122,7 @@ def write_tune_file(entries, path=TUNE_FILE):
                 lines[int(line.split()[0])] = line.strip()
     for n, ids in entries.items():
         if ids:
-            lines[int(n)] = f"{n} " + " ".join(f"{vs[i]['R1']}x{vs[i]['R2']}/{vs[i]['T']}" for i in ids)
+            lines[int(n)] = f"{n} " + " ".join(tune_key(vs[i]) for i in ids)
         else:
             lines.pop(int(n), None)
     with open(path, "w") as f:

@@ -1,5 +1,6 @@
 // C ABI (include/goldilocks_gpu.h): host-buffer and device-buffer entry
 // points for the NTT, K12 layers, Merkle trees and the commit.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -120,6 +121,27 @@ int gl_init(int device) {
 }
 
 int gl_is_divisor(uint64_t n) { return n > 0 && (P - 1) % n == 0; }
+
+uint64_t gl_next_size(uint64_t n) {
+    // the 1056 divisors of p - 1 = 2^32 * 3 * 5 * 17 * 257 * 65537, sorted
+    static const std::vector<uint64_t> divs = [] {
+        std::vector<uint64_t> d;
+        const uint64_t odd[] = {3, 5, 17, 257, 65537};
+        for (int m = 0; m < 32; m++) {
+            for (int k = 0; k <= 32; k++) {
+                uint64_t v = 1ull << k;
+                for (int q = 0; q < 5; q++)
+                    if (m & (1 << q)) v *= odd[q];
+                d.push_back(v);
+            }
+        }
+        std::sort(d.begin(), d.end());
+        return d;
+    }();
+    if (n == 0) return 0;
+    auto it = std::lower_bound(divs.begin(), divs.end(), n);
+    return it == divs.end() ? 0 : *it;
+}
 
 // ---------------------------------------------------------------- NTT
 

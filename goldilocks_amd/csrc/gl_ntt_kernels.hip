@@ -146,14 +146,15 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
                 ka = tb / T;
             }
             const u32 I = I0 + (u32)t;
+            const bool valid = I < a.total_inst;
+            u32 b = 0, i = 0, s = 0, m = 0;
+            if (valid) locate(a, I, b, i, s, m);
             u64 z[R2];
             const u64 *row = lds + t * TS + ka * RS;
 #pragma unroll
             for (int rb = 0; rb < R2; rb++) z[rb] = row[rb];
             dft<R2>(z);
-            if (I < a.total_inst) {
-                u32 b, i, s, m;
-                locate(a, I, b, i, s, m);
+            if (valid) {
                 u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
                 store_outputs<R2, R1>(a, dst, a.S * (u64)R1, m, (u32)ka, (u32)R, z);
             }

@@ -46,6 +46,11 @@ int gl_init(int device);
 /* ntt/src/divisors.rs:21-23: n > 0 and n | p - 1. */
 int gl_is_divisor(uint64_t n);
 
+/* ntt/src/divisors.rs:52-59 (divisors::next): the smallest supported NTT
+ * length >= n; 0 for n = 0, and 0 when n > p - 1 (where the reference's
+ * lookup runs off its table).  Host-only. */
+uint64_t gl_next_size(uint64_t n);
+
 /* ---- NTT, host buffers (copied to HBM and back; synchronous) ------------- */
 
 /* Forward transform in place.  Replaces goldilocks_ntt::ntt::ntt

@@ -42,3 +42,17 @@ def test_pure_host_helpers():
     assert gl.is_divisor(1 << 32) and not gl.is_divisor(7) and not gl.is_divisor(0)
     with pytest.raises(ValueError):
         gl.strategy(7)
+
+
+def test_next_size_matches_reference_kats(oracle):
+    """divisors::next KATs (ntt/src/divisors.rs:110-115) and agreement with
+    the oracle's restatement over a sweep (host-only entry point)."""
+    import goldilocks_amd as gl
+    assert gl.next_size(0) == 0
+    assert gl.next_size(1000) == 1020
+    assert gl.next_size(1024) == 1024
+    assert gl.next_size(1234) == 1280
+    assert gl.next_size(0xFFFFFFFF00000000) == 0xFFFFFFFF00000000
+    assert gl.next_size(0xFFFFFFFF00000001) == 0
+    for n in list(range(1, 5000, 7)) + [3 ** 7, 10 ** 6, 10 ** 9, 2 ** 40 + 1, 2 ** 62 + 3]:
+        assert gl.next_size(n) == oracle.next_divisor(n), n

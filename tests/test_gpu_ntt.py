@@ -109,7 +109,7 @@ def test_errors(gl):
 def test_device_columns(gl, oracle):
     """Column transforms of a row-major matrix (the commit's layout)."""
     import torch
-    for rows, cols in ((1024, 64), (4096, 3), (2**14, 8), (240, 5)):
+    for rows, cols in ((1024, 64), (4096, 3), (2**14, 8), (240, 5), (2**18, 16), (257 * 256, 4), (3 * 2**15, 7)):
         note(f"columns {rows}x{cols}")
         m = oracle.fill_random(rows * cols, rows + cols).reshape(rows, cols)
         t = to_torch(m)
@@ -152,3 +152,29 @@ def test_large_device(gl, oracle, k):
     assert torch.equal(y, x)
     if k <= 20:
         assert np.array_equal(host_y, oracle.ntt(host_x))
+
+
+def test_pass_timing_records_every_pass(gl):
+    """gl_timing_enable/collect: one record per pass launch, in launch order,
+    named like the schedule, none while disabled."""
+    import torch
+    n = 1 << 16
+    x = torch.empty(n, dtype=torch.int64, device="cuda")
+    gl.fill_random(x, 3)
+    plan = [p["kernel"] for p in gl.plan_info(n)]
+    note(f"pass timing n={n} plan={plan}")
+    gl.timing_collect()
+    gl.ntt(x)
+    torch.cuda.synchronize()
+    assert gl.timing_collect() == []
+    gl.timing_enable(True)
+    try:
+        for _ in range(3):
+            gl.ntt(x)
+        torch.cuda.synchronize()
+    finally:
+        gl.timing_enable(False)
+    recs = gl.timing_collect()
+    assert [name for name, _ in recs] == plan * 3
+    assert all(ms > 0 for _, ms in recs)
+    assert gl.timing_collect() == []
