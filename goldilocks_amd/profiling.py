@@ -71,11 +71,38 @@ def write_stats_md(f, args, stats):
         f.write(f"| `{short(name)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |\n")
 
 
+def counters_md(f, path, kernel_filter):
+    """Per-kernel averages of every counter in a --pmc run, plus the VALU
+    issue fraction: SQ_INSTS_VALU wave instructions x 4 cycles (wave64 on a
+    16-lane SIMD) over 1024 SIMDs, against SQ_BUSY_CYCLES / 32 (summed over
+    the 32 shader engines of 8 XCDs)."""
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            if kernel_filter not in r["Kernel_Name"]:
+                continue
+            k = short(r["Kernel_Name"])
+            per[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[k].add(int(r["Dispatch_Id"]))
+    for k, c in per.items():
+        nd = len(disp[k])
+        names = sorted(c)
+        f.write(f"\n`{k}`, {nd} dispatches, per-dispatch averages:\n\n")
+        f.write("| " + " | ".join(names) + " |\n|" + "---|" * len(names) + "\n")
+        f.write("| " + " | ".join(f"{c[n] / nd:.0f}" for n in names) + " |\n")
+        if "SQ_INSTS_VALU" in c and "SQ_BUSY_CYCLES" in c:
+            valu = c["SQ_INSTS_VALU"] / nd * 4 / 1024
+            busy = c["SQ_BUSY_CYCLES"] / nd / 32
+            f.write(f"\nVALU issue: {valu:.0f} cycles per SIMD over {busy:.0f} busy cycles = "
+                    f"**{valu / busy:.2f}** of peak\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", required=True)
     ap.add_argument("--copy-bytes", type=int, default=0)
-    ap.add_argument("--stats", required=True)
+    ap.add_argument("--stats", default=None)
     ap.add_argument("--fetch", default=None)
     ap.add_argument("--write", default=None)
     ap.add_argument("--algorithmic-bytes", type=int, default=None,
@@ -83,7 +110,18 @@ def main():
     ap.add_argument("--json", default=None)
     ap.add_argument("--md", required=True)
     ap.add_argument("--command", default="")
+    ap.add_argument("--counters", default=None, help="counter_collection.csv of a --pmc run")
+    ap.add_argument("--kernel", default="ntt_pass", help="kernel-name filter for --counters")
     args = ap.parse_args()
+    if args.counters:
+        with open(args.md, "w") as f:
+            f.write(f"# rocprofv3 counters: {args.workload}\n\n")
+            if args.command:
+                f.write("Command (repo root, one MI355X):\n\n")
+                for c in args.command.split(";;"):
+                    f.write(f"    {c.strip()}\n")
+            counters_md(f, args.counters, args.kernel)
+        return
     stats = kernel_stats(args.stats)
     if not (args.fetch and args.write):
         with open(args.md, "w") as f:
