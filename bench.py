@@ -25,8 +25,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve"])
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
@@ -112,16 +112,24 @@ def bench_ntt(args, torch, gl):
     for _ in range(args.warmup):
         gl.ntt(x)
     torch.cuda.synchronize()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    # Timed region: exactly K steps, synchronized on both sides, bracketed by
+    # two events on the launch stream.
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        gl.ntt(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # Per-step latency: the same steps with an event after each one.
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     evs[0].record(stream)
     for i in range(args.steps):
         gl.ntt(x)
         evs[i + 1].record(stream)
     torch.cuda.synchronize()
     per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
-    total_ms = evs[0].elapsed_time(evs[-1])
-    ms = total_ms / args.steps
     per_sorted = sorted(per)
     p50 = per_sorted[len(per) // 2]
     p99 = per_sorted[min(len(per) - 1, int(0.99 * len(per)))]

@@ -104,6 +104,9 @@ def lib():
             "gl_k12_layer_dev": (i32, [vp, sz, vp, vp]),
             "gl_merkle_tree": (i32, [vp, sz, vp, sz, psz]),
             "gl_merkle_tree_dev": (i32, [vp, sz, vp, sz, psz, vp]),
+            "gl_merkle_depth": (sz, [sz]),
+            "gl_merkle_open_dev": (i32, [vp, sz, vp, vp, sz, vp, vp]),
+            "gl_merkle_verify_dev": (i32, [vp, sz, vp, sz, vp, vp, vp]),
             "gl_commit": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz]),
             "gl_commit_dev": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz, vp]),
         }
@@ -142,6 +145,13 @@ def _host_u64(values):
 def _dev_ptr(t):
     if not t.is_cuda or t.element_size() != 8 or not t.is_contiguous():
         raise TypeError("expected a contiguous 64-bit torch tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _bytes_dev_ptr(t):
+    """Device address of a contiguous GPU tensor of any dtype."""
+    if not t.is_cuda or not t.is_contiguous():
+        raise TypeError("expected a contiguous torch tensor on the GPU")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -457,6 +467,39 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def merkle_depth(leaf_len):
+    """Number of hashed layers of a tree over leaf_len bytes."""
+    return lib().gl_merkle_depth(leaf_len)
+
+
+def merkle_open(leaves, layers, blocks):
+    """Openings of leaf blocks `blocks` (int64 GPU tensor of 160-byte block
+    indices) from a device-resident tree: `leaves` (any contiguous GPU
+    tensor, its bytes are the leaves) and `layers` (the uint8 tensor
+    merkle_tree returned for it).  Returns a uint8 GPU tensor of shape
+    (len(blocks), depth, 160): for each layer l, block blocks[i] // 5**l of
+    layer l, zero padded."""
+    import torch
+    leaf_len = leaves.numel() * leaves.element_size()
+    depth = merkle_depth(leaf_len)
+    k = blocks.numel()
+    out = torch.zeros((k, depth, 160), dtype=torch.uint8, device=leaves.device)
+    _check(lib().gl_merkle_open_dev(_bytes_dev_ptr(leaves), leaf_len, _bytes_dev_ptr(layers), _dev_ptr(blocks), k,
+                                    _bytes_dev_ptr(out), _stream_ptr(leaves)))
+    return out
+
+
+def merkle_verify(paths, leaf_len, blocks, root):
+    """Check openings (layout of merkle_open) against a 32-byte root (uint8
+    GPU tensor); returns a bool GPU tensor, one flag per opening."""
+    import torch
+    k = blocks.numel()
+    ok = torch.zeros(k, dtype=torch.int32, device=paths.device)
+    _check(lib().gl_merkle_verify_dev(_bytes_dev_ptr(paths), leaf_len, _dev_ptr(blocks), k, _bytes_dev_ptr(root),
+                                      _bytes_dev_ptr(ok), _stream_ptr(paths)))
+    return ok.bool()
 
 
 # ---------------------------------------------------------------- serving

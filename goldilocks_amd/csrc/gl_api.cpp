@@ -328,6 +328,59 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     return GL_OK;
 }
 
+// ---------------------------------------------------------------- openings
+
+static int merkle_geometry(const uint8_t *leaves, size_t leaf_len, const uint8_t *layers, MerkleGeometry &g) {
+    g.depth = 0;
+    size_t len = leaf_len, off = 0;
+    const uint8_t *p = leaves;
+    while (len > HASH) {
+        if (g.depth == kMaxMerkleDepth) return set_error(GL_E_UNSUPPORTED, "Merkle tree too deep");
+        g.base[g.depth] = p;
+        g.len[g.depth] = len;
+        g.depth++;
+        len = next_layer(len);
+        p = layers ? layers + off : nullptr;
+        off += len;
+    }
+    g.base[g.depth] = p;
+    g.len[g.depth] = len;
+    return GL_OK;
+}
+
+size_t gl_merkle_depth(size_t leaf_len) {
+    size_t d = 0;
+    for (size_t len = leaf_len; len > HASH; len = next_layer(len)) d++;
+    return d;
+}
+
+int gl_merkle_open_dev(const uint8_t *d_leaves, size_t leaf_len, const uint8_t *d_layers, const uint64_t *d_blocks,
+                       size_t k, uint8_t *d_out, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    MerkleGeometry g;
+    if ((rc = merkle_geometry(d_leaves, leaf_len, d_layers, g))) return rc;
+    if (k == 0 || g.depth == 0) return GL_OK;
+    if (!d_leaves || !d_blocks || !d_out || (g.depth > 1 && !d_layers)) return set_error(GL_E_ARG, "null device pointer");
+    launch_merkle_open(g, d_blocks, k, d_out, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int gl_merkle_verify_dev(const uint8_t *d_paths, size_t leaf_len, const uint64_t *d_blocks, size_t k,
+                         const uint8_t *d_root32, int32_t *d_ok, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    MerkleGeometry g;
+    if ((rc = merkle_geometry(nullptr, leaf_len, nullptr, g))) return rc;
+    if (k == 0) return GL_OK;
+    if (g.depth == 0) return set_error(GL_E_ARG, "a tree of at most 32 bytes has no openings");
+    if (!d_paths || !d_blocks || !d_root32 || !d_ok) return set_error(GL_E_ARG, "null device pointer");
+    launch_merkle_verify(g, d_paths, d_blocks, k, d_root32, (int *)d_ok, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
 int gl_merkle_tree_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
                        void *stream) {
     int rc = ensure_init();

@@ -115,6 +115,19 @@ void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blow
 // stage: 21 words of device scratch (the padded partial last block)
 void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage, hipStream_t s);
 void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hipStream_t s);
+
+// Merkle tree geometry for openings: layer l has len[l] bytes at base[l]
+// (l = 0: the leaves); depth = number of hashed layers (the root layer,
+// 32 bytes, is layer depth).
+constexpr int kMaxMerkleDepth = 48;
+struct MerkleGeometry {
+    const uint8_t *base[kMaxMerkleDepth + 1];
+    u64 len[kMaxMerkleDepth + 1];
+    int depth;
+};
+void launch_merkle_open(const MerkleGeometry &g, const u64 *blocks, u64 k, uint8_t *out, hipStream_t s);
+void launch_merkle_verify(const MerkleGeometry &g, const uint8_t *paths, const u64 *blocks, u64 k,
+                          const uint8_t *root32, int *ok, hipStream_t s);
 void launch_pow_search(const uint8_t *challenge32, u64 threshold, u64 start, u64 count,
                        unsigned long long *best, hipStream_t s);
 

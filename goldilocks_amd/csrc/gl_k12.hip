@@ -264,6 +264,62 @@ void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hip
     hipLaunchKernelGGL(k12_single_kernel, dim3(1), dim3(64), 0, s, in, len, out, outlen);
 }
 
+// ---- Merkle openings.  Layer l (l = 0: the leaf bytes) has len[l] bytes and
+// starts at base[l]; an opening of leaf block b holds, for every hashed layer
+// l < depth, the block b_l = b / 5^l of layer l (160 bytes, zero padded past
+// the end of the layer): hashing it gives entry b_l of layer l + 1, which
+// sits in block b_{l+1} at byte 32 * (b_l % 5).
+
+// out[(i * depth + l) * 160 + j], one thread per byte.
+__global__ void merkle_open_kernel(MerkleGeometry g, const u64 *__restrict__ blocks, u64 k, uint8_t *__restrict__ out) {
+    const u64 total = k * (u64)g.depth * 160;
+    for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
+        const u64 j = t % 160, il = t / 160, l = il % g.depth, i = il / g.depth;
+        u64 b = blocks[i];
+        for (u64 q = 0; q < l; q++) b /= 5;
+        const u64 pos = b * 160 + j;
+        out[t] = pos < g.len[l] ? g.base[l][pos] : 0;
+    }
+}
+
+// ok[i] = 1 when opening i of leaf block blocks[i] hashes up to root32.
+__global__ void merkle_verify_kernel(MerkleGeometry g, const uint8_t *__restrict__ paths, const u64 *__restrict__ blocks,
+                                     u64 k, const uint8_t *__restrict__ root32, int *__restrict__ ok) {
+    for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < k; i += (u64)gridDim.x * blockDim.x) {
+        u64 b = blocks[i];
+        bool good = b * 160 < g.len[0];
+        for (int l = 0; l < g.depth && good; l++) {
+            const uint8_t *blk = paths + (i * (u64)g.depth + l) * 160;
+            const u64 rem = g.len[l] - b * 160, bl = rem < 160 ? rem : 160;
+            u64 a[25];
+            absorb_padded(a, blk, bl);
+            keccak_p12(a);
+            const uint8_t *want = l + 1 < g.depth ? paths + (i * (u64)g.depth + l + 1) * 160 + 32 * (b % 5) : root32;
+            for (int w = 0; w < 4 && good; w++)
+                for (int q = 0; q < 8; q++)
+                    if ((uint8_t)(a[w] >> (8 * q)) != want[8 * w + q]) good = false;
+            b /= 5;
+        }
+        ok[i] = good ? 1 : 0;
+    }
+}
+
+void launch_merkle_open(const MerkleGeometry &g, const u64 *blocks, u64 k, uint8_t *out, hipStream_t s) {
+    const u64 total = k * (u64)g.depth * 160;
+    if (total == 0) return;
+    u64 grid = (total + 255) / 256;
+    hipLaunchKernelGGL(merkle_open_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(256), 0, s, g, blocks,
+                       k, out);
+}
+
+void launch_merkle_verify(const MerkleGeometry &g, const uint8_t *paths, const u64 *blocks, u64 k,
+                          const uint8_t *root32, int *ok, hipStream_t s) {
+    if (k == 0) return;
+    u64 grid = (k + 63) / 64;
+    hipLaunchKernelGGL(merkle_verify_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(64), 0, s, g, paths,
+                       blocks, k, root32, ok);
+}
+
 void launch_pow_search(const uint8_t *challenge32, u64 threshold, u64 start, u64 count,
                        unsigned long long *best, hipStream_t s) {
     u64 g = (count + 255) / 256;

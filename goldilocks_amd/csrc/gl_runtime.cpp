@@ -762,8 +762,15 @@ void timing_enable(bool on) { g_tm_on.store(on); }
 
 // Brackets one launch: call with e0 = timing_begin(s) before and
 // timing_end(v, e0, s) after.
+// Records kept until collected; beyond this many, launches go untimed.
+constexpr size_t kMaxTimingRecords = size_t(1) << 20;
+
 hipEvent_t timing_begin(hipStream_t s) {
     if (!g_tm_on.load()) return nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_tm_mu);
+        if (g_tm_recs.size() >= kMaxTimingRecords) return nullptr;
+    }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
         (void)hipGetLastError();

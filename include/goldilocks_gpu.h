@@ -140,7 +140,8 @@ uint64_t gl_set_twiddle_full_max(uint64_t words);
  * gl_timing_collect waits for them and returns, in launch order, the variant
  * id (as for gl_variant_name) and duration in ms of up to cap of them;
  * *count = number recorded (all are released).  Launches made while a
- * stream is being captured into a graph are not timed. */
+ * stream is being captured into a graph, or once 2^20 records await
+ * collection, are not timed. */
 void gl_timing_enable(int on);
 size_t gl_timing_pending(void);
 int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *count);
@@ -182,6 +183,30 @@ int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_no
  * on the raw words.  Length n must divide p - 1. */
 int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n);
 int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stream);
+
+/* ---- Merkle openings (the step after the commit) --------------------------
+ * No reference function: the reference returns the layers (pcs/src/k12/
+ * mod.rs:41-60) and stops.  An opening of leaf block b (the 160-byte block
+ * b of the leaf bytes) is depth blocks of 160 bytes: for layer l = 0 (the
+ * leaves) .. depth-1, block b / 5^l of layer l, zero padded past the end of
+ * the layer.  Hashing it (k12::<32> of its unpadded bytes) gives the entry
+ * that sits at byte 32 * ((b / 5^l) % 5) of the next block; the last one
+ * hashes to the 32-byte root. */
+
+/* Number of hashed layers of a tree over leaf_len bytes (= the number of
+ * layers merkle_tree returns). */
+size_t gl_merkle_depth(size_t leaf_len);
+
+/* k openings of leaf blocks d_blocks[i] from a device-resident tree (leaf
+ * bytes + the layers gl_merkle_tree_dev wrote) into d_out, k * depth * 160
+ * bytes.  Block indices past the leaves give all-zero openings. */
+int gl_merkle_open_dev(const uint8_t *d_leaves, size_t leaf_len, const uint8_t *d_layers, const uint64_t *d_blocks,
+                       size_t k, uint8_t *d_out, void *stream);
+
+/* d_ok[i] = 1 when opening i (layout of gl_merkle_open_dev) of leaf block
+ * d_blocks[i] hashes up to d_root32, else 0. */
+int gl_merkle_verify_dev(const uint8_t *d_paths, size_t leaf_len, const uint64_t *d_blocks, size_t k,
+                         const uint8_t *d_root32, int32_t *d_ok, void *stream);
 
 /* ---- Commit (no single reference function; composed from ntt::intt,
  * coefficient scaling by GENERATOR^i, ntt::ntt and merkle_tree over the raw
