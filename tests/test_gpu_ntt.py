@@ -154,6 +154,43 @@ def test_large_device(gl, oracle, k):
         assert np.array_equal(host_y, oracle.ntt(host_x))
 
 
+def _sum_mod_p(t):
+    """Exact sum of u64 words (an int64 GPU tensor, < 2^31 elements) mod p:
+    32-bit halves summed in int64 cannot overflow."""
+    import torch
+    lo = (t & 0xFFFFFFFF).sum().item()
+    hi = torch.bitwise_right_shift(t, 32).bitwise_and(0xFFFFFFFF).sum().item()
+    return (hi * (1 << 32) + lo) % P
+
+
+@pytest.mark.parametrize("k", [30, 31])
+def test_max_size_device(gl, k):
+    """Lengths where word positions pass 2^30 / 2^31 and byte offsets pass
+    2^33 / 2^34 (8 and 16 GiB): X[0] = sum x_j and X[n/2] = sum (-1)^j x_j
+    exactly, a run is deterministic, and the roundtrip is exact."""
+    import torch
+    n = 1 << k
+    free, _ = torch.cuda.mem_get_info()
+    if free < 4 * 8 * n:
+        pytest.skip(f"needs {4 * 8 * n >> 30} GiB of device memory")
+    note(f"max size n=2^{k}")
+    x = torch.empty(n, dtype=torch.int64, device="cuda")
+    gl.fill_random(x, 7)
+    y = x.clone()
+    gl.ntt(y)
+    torch.cuda.synchronize()
+    assert int(y[0].item()) % (1 << 64) == _sum_mod_p(x)
+    even, odd = _sum_mod_p(x[0::2].contiguous()), _sum_mod_p(x[1::2].contiguous())
+    assert int(y[n // 2].item()) % (1 << 64) == (even - odd) % P
+    z = x.clone()
+    gl.ntt(z)
+    assert torch.equal(y, z)
+    del z
+    gl.intt(y)
+    torch.cuda.synchronize()
+    assert torch.equal(y, x)
+
+
 def test_pass_timing_records_every_pass(gl):
     """gl_timing_enable/collect: one record per pass launch, in launch order,
     named like the schedule, none while disabled."""
