@@ -100,11 +100,21 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     // intt of the columns, reading the input matrix in the first pass
     if (prow->passes.empty()) GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
     if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m))) return rc;
-    PowTable g = power_table(2717, rows);
-    if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-    launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
-    GL_HIP(hipGetLastError());
-    if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s))) return rc;
+    // extension: the coset shift by 2717^j and the zero rows j >= rows are
+    // folded into the first pass when it is a two-sub-step kernel (it then
+    // reads coef directly); otherwise a separate kernel builds the input.
+    const bool fold = !pext->passes.empty() && pext->passes[0].v->kind == VK_TWO_STAGE;
+    const u64 *ct = fold ? coset_table(rows) : nullptr;
+    if (fold && !ct) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    if (fold) {
+        if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, ct, rows))) return rc;
+    } else {
+        PowTable g = power_table(2717, rows);
+        if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+        launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
+        GL_HIP(hipGetLastError());
+        if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s))) return rc;
+    }
     size_t need = tree_size(ext * cols * 8);
     return merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, need, nullptr, s);
 }

@@ -34,6 +34,8 @@ struct PassArgs {
     const u64 *tw_lo;
     const u64 *tw_hi;
     u64 scale;         // multiplier applied to every output when FLAG_SCALE
+    const u64 *coset;  // FLAG_COSET: input row j < coset_rows is multiplied by coset[j]
+    u64 coset_rows;    //   (Montgomery g^j); rows >= coset_rows read as zero
     u32 flags;
     // x / ninst and x / S for x < 2^31 as (x * mul) >> sh (fastdiv_params)
     u32 ninst_mul, ninst_sh, S_mul, S_sh;
@@ -72,6 +74,7 @@ enum : u32 {
     FLAG_REVERSE = 2u,  // read x[(n - j) mod n] (first pass of intt)
     FLAG_SCALE = 4u,    // multiply outputs by `scale` (1/n for intt)
     FLAG_TWIDDLE_FULL = 8u,  // tw_lo is a full table: output k of instance m uses tw_lo[k + R*m]
+    FLAG_COSET = 16u,  // first pass of the commit's extension: coset shift + zero rows
 };
 // All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
 // are applied with mont_mul.
@@ -108,6 +111,7 @@ void launch_rader_big_gather(const PassArgs &a, const u32 *permk, u64 *C, u64 *x
 void launch_rader_big_pointwise(u64 *C, u64 *x0, const u64 *spec, u64 ninst, hipStream_t s);
 void launch_rader_big_scatter(const PassArgs &a, const u32 *permi, const u64 *D, const u64 *X0, hipStream_t s);
 void launch_pointwise_mont(u64 *a, const u64 *b, u64 n, hipStream_t s);
+void launch_power_series(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s);
 void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 

@@ -104,6 +104,16 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
                     u64 rpos = pos ? a.n_rev - pos : 0;
                     x[ra] = src[s + a.S * rpos];
                 }
+            } else if (a.flags & FLAG_COSET) {
+                // input row j = m + Mp*r of a zero-extended, coset-shifted
+                // column: coef[j] * g^j below coset_rows, zero above
+                const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+                const u64 step = a.SMp * (u64)R2;
+#pragma unroll
+                for (int ra = 0; ra < R1; ra++) {
+                    const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+                    x[ra] = j < a.coset_rows ? mont_mul(p[step * (u64)ra], a.coset[j]) : 0;
+                }
             } else {
                 const u64 *p = src + (u64)i + a.SMp * (u64)rb;
                 const u64 step = a.SMp * (u64)R2;
@@ -430,6 +440,19 @@ void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 
     unsigned grid = (unsigned)((count + 255) / 256 < 65536 ? (count + 255) / 256 : 65536);
     if (grid == 0) return;
     hipLaunchKernelGGL(twiddle_matrix_kernel, dim3(grid), dim3(256), 0, s, out, count, R, lo, hi, lo_bits);
+}
+
+// out[j] = lo[j & mask] * hi[j >> lo_bits] (Montgomery forms): the powers of
+// a table's base for j < count.
+__global__ void power_series_kernel(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits) {
+    for (u64 j = blockIdx.x * (u64)blockDim.x + threadIdx.x; j < count; j += (u64)gridDim.x * blockDim.x)
+        out[j] = mont_mul(lo[j & ((1ull << lo_bits) - 1)], hi[j >> lo_bits]);
+}
+
+void launch_power_series(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s) {
+    unsigned grid = (unsigned)((count + 255) / 256 < 65536 ? (count + 255) / 256 : 65536);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(power_series_kernel, dim3(grid), dim3(256), 0, s, out, count, lo, hi, lo_bits);
 }
 
 // ---- radix-65537 Rader pass pieces (rader.rs:65-93 with (gi, gk) = (3, 21846)).

@@ -276,6 +276,31 @@ const u64 *inner_table(u64 R) {
     return d;
 }
 
+// Montgomery g^j for j < rows, g = 2717 (the commit's coset shift, applied
+// by the first pass of the extension).  rows words, cached per rows.
+static std::map<u64, const u64 *> g_coset_tables;
+const u64 *coset_table(u64 rows) {
+    PowTable t = power_table(2717, rows);
+    if (!t.lo || !t.hi) return nullptr;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_coset_tables.find(rows);
+    if (it != g_coset_tables.end()) return it->second;
+    void *d = nullptr;
+    if (hipMalloc(&d, rows * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    launch_power_series((u64 *)d, rows, t.lo, t.hi, t.lo_bits, nullptr);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_tw_allocs.push_back(d);
+    g_coset_tables[rows] = (const u64 *)d;
+    return (const u64 *)d;
+}
+
 // Inner twiddles of a two-sub-step pass R = R1 x R2, ordered so that one
 // thread's R1 - 1 multipliers are consecutive: entry rb*R1 + ka =
 // w_R^(ka*rb) (Montgomery form).
@@ -709,6 +734,11 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         }
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
+    if (coset && p == 0) {
+        a.flags |= FLAG_COSET;
+        a.coset = coset;
+        a.coset_rows = coset_rows;
+    }
     if (inverse && last && scale_out) {
         a.flags |= FLAG_SCALE;
         a.scale = to_mont(inv(n % P));
@@ -827,11 +857,13 @@ int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count) {
 }
 
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream, bool scale_out, const u64 *input) {
+             hipStream_t stream, bool scale_out, const u64 *input, const u64 *coset, u64 coset_rows) {
     if (plan.passes.empty() || count == 0) return GL_OK;
     PassCursor c{&plan, data, tmp, S0, count, bstride, inverse};
     c.scale_out = scale_out;
     c.input = input;
+    c.coset = coset;
+    c.coset_rows = coset_rows;
     c.begin();
     while (!c.done()) {
         const PassVariant *v = plan.passes[c.p].v;
@@ -865,6 +897,7 @@ void clear_caches() {
         g_pow_tables.clear();
         g_inner_tables.clear();
         g_inner_matrices.clear();
+        g_coset_tables.clear();
         g_full_tables.clear();
         g_rader257 = nullptr;
         g_big_rader = BigRaderTables();

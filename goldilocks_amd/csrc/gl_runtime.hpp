@@ -92,6 +92,10 @@ struct PassCursor {
     bool inverse;
     bool scale_out = true;  // inverse: multiply the last pass by 1/n
     const u64 *input = nullptr;  // first pass reads here instead of data (out of place)
+    // first pass only (two-sub-step variants): input row j < coset_rows is
+    // multiplied by coset[j], rows above read as zero (FLAG_COSET)
+    const u64 *coset = nullptr;
+    u64 coset_rows = 0;
     // evolving state
     size_t p = 0;
     u64 S = 0, M = 0;
@@ -108,7 +112,10 @@ struct PassCursor {
 // input (optional): the first pass reads the transform from there and the
 // result lands in data (same layout), leaving input untouched.
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream, bool scale_out = true, const u64 *input = nullptr);
+             hipStream_t stream, bool scale_out = true, const u64 *input = nullptr, const u64 *coset = nullptr,
+             u64 coset_rows = 0);
+// Montgomery 2717^j for j < rows (device, cached).
+const u64 *coset_table(u64 rows);
 // True when the plan contains a pass that is not a single kernel (the
 // radix-65537 Rader pass), which grouped batch launches cannot include.
 bool plan_is_composite(const Plan &plan);
