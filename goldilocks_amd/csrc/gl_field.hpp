@@ -43,20 +43,37 @@ GL_HD u64 add_carry(u64 a, u64 b, u32 &c) {
     return mk64(r0, r1);
 }
 
-// r + k * 2^64 reduced to [0, p) for k in {0, 1}, given that the value is
-// below 2^64 + p (then one + (2^32 - 1) step suffices: either the sum
-// carried, or it did not and r >= p, which is exactly when r + EPS carries).
-GL_HD u64 canon_carry(u64 r, u32 k) {
-    u32 k2;
-    u64 t = add_carry(r, EPS, k2);
-    return (k | k2) ? t : r;
+// Keeps a value opaque to the optimizer (no instruction): a product with
+// it stays one v_mad instead of being rewritten as selects.
+GL_HD int32_t opaque(int32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(x));
+#endif
+    return x;
 }
 
-// a + b for canonical a, b: same result as generic.rs:17-26.
+// Same for a 64-bit value, which also keeps it in one register pair.
+GL_HD u64 opaque64(u64 x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(x));
+#endif
+    return x;
+}
+
+// r + k (2^32 - 1) mod 2^64 for k in {0, 1}: one v_mad_u64_u32.
+GL_HD u64 plus_eps(u64 r, bool k) { return opaque64(r) + (u64)(u32)opaque(k ? 1 : 0) * EPS; }
+
+// r - k (2^32 - 1) = r + k p mod 2^64 for k in {0, 1}: one v_mad_i64_i32,
+// with 2^32 - 1 = 65535 * 65537.
+GL_HD u64 minus_eps(u64 r, bool k) { return opaque64(r) + (u64)((int64_t)opaque(k ? -65535 : 0) * 65537); }
+
+// a + b for canonical a, b: same result as generic.rs:17-26.  a + b >= p
+// exactly when the sum carried (then s + EPS < p) or s >= p (then s + EPS
+// wraps to s - p).
 GL_HD u64 add(u64 a, u64 b) {
     u32 k;
     u64 s = add_carry(a, b, k);
-    return canon_carry(s, k);
+    return plus_eps(s, k | (s >= P));
 }
 
 // a - b for canonical a, b (generic.rs:31-39): on borrow add p, which is
@@ -64,7 +81,7 @@ GL_HD u64 add(u64 a, u64 b) {
 GL_HD u64 sub(u64 a, u64 b) {
     u32 c;
     u64 r = sub_borrow(a, b, c);
-    return c ? r + P : r;
+    return minus_eps(r, c);
 }
 
 GL_HD u64 neg(u64 a) { return a ? P - a : 0; }
@@ -110,7 +127,7 @@ GL_HD u64 mont_mul(u64 a, u64 b) {
     u32 t0 = __builtin_subc(lo32(x0), s1, e, &c1);
     u32 t1 = __builtin_subc(s1, 0u, c1, &c2);
     u64 r = sub_borrow(x1, mk64(t0, t1), bw);
-    return bw ? r + P : r;
+    return minus_eps(r, bw);
 }
 
 // Montgomery form of a canonical value: a * 2^64 mod p.
@@ -127,28 +144,29 @@ GL_HD u64 shl_lt96(u64 a) {
         // H < 2^32: value = L + H * EPS, one carry at most.
         u32 k;
         u64 r = add_carry((u64)(u32)(a >> (64 - S)) * EPS, a << S, k);
-        return canon_carry(r, k);
+        return plus_eps(r, k | (r >= P));
     } else if constexpr (S < 64) {
-        // H = h0 + h1 * 2^32: value = L + h0 * EPS - h1.
-        u64 H = a >> (64 - S);
-        u32 k, b;
-        u64 r = add_carry((u64)lo32(H) * EPS, a << S, k);
-        u64 d = sub_borrow(r, (u64)hi32(H), b);
-        // value = d + (k - b) * 2^64.  k = 1, b = 0 gives d + EPS < p;
-        // k = 0, b = 1 gives d - EPS; k = b needs d reduced (k = b = 1
-        // implies d >= p).
-        u32 plus = (u32)(d >= P) ^ k ^ b, minus = (k ^ 1u) & b;
-        return d + mk64(plus ? 0xffffffffu : minus, minus ? 0xffffffffu : 0u);
+        // H = h0 + h1 2^32 < 2^S, L = a << S has a zero low word:
+        // value = (Lh + h0) 2^32 - h0 - h1.  With u0 + c 2^32 = Lh + h0,
+        // X = {c ? EPS : 0, u0} equals (Lh + h0) 2^32 mod p and X < p, so
+        // X - h0 - h1 (h0 + h1 < 2^33) needs at most one + p.
+        const u32 a0 = lo32(a), a1 = hi32(a);
+        const u32 h0 = (u32)(a >> (64 - S)), h1 = a1 >> (64 - S);
+        u32 c, b1, b2;
+        const u32 u0 = __builtin_addc(a0 << (S - 32), h0, 0u, &c);
+        u64 r = sub_borrow(mk64(c ? 0xffffffffu : 0u, u0), (u64)h0, b1);
+        r = sub_borrow(r, (u64)h1, b2);
+        return minus_eps(r, b1 | b2);
     } else if constexpr (S == 64) {
         // value = a0 * EPS - a1
         u32 b;
         u64 d = sub_borrow((u64)lo32(a) * EPS, (u64)hi32(a), b);
-        return b ? d + P : d;
+        return minus_eps(d, b);
     } else {
         // limbs c2 = bits [64, 96), u = bits [96, 160): value = c2 * EPS - u
         u32 b;
         u64 d = sub_borrow((u64)(u32)(a << (S - 64)) * EPS, a >> (96 - S), b);
-        return b ? d + P : d;
+        return minus_eps(d, b);
     }
 }
 
