@@ -97,16 +97,25 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     int rc;
     if ((rc = cols > 1 ? get_plan_cols(rows, &prow) : get_plan(rows, &prow))) return rc;
     if ((rc = cols > 1 ? get_plan_cols(ext, &pext) : get_plan(ext, &pext))) return rc;
+    // extension: the coset shift by 2717^j and the zero rows j >= rows are
+    // folded into the first extension pass when it is a two-sub-step kernel
+    // (it then reads coef directly); otherwise a separate kernel builds the
+    // input.  When the intt's last pass is a single kernel as well, the
+    // shift joins the intt's 1/rows scale (one product per output either
+    // way) and the extension pass only zero-extends.
+    const bool fold = !pext->passes.empty() && pext->passes[0].v->kind == VK_TWO_STAGE;
+    const int last_kind = prow->passes.empty() ? (int)VK_COMPOSITE : prow->passes.back().v->kind;
+    const bool fold_scale = fold && (last_kind == VK_TWO_STAGE || last_kind == VK_THREE_STAGE);
+    const u64 *ct = fold && !fold_scale ? coset_table(rows) : nullptr;
+    const u64 *sct = fold_scale ? scaled_coset_table(rows) : nullptr;
+    if ((fold && !fold_scale && !ct) || (fold_scale && !sct))
+        return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     // intt of the columns, reading the input matrix in the first pass
     if (prow->passes.empty()) GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
-    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m))) return rc;
-    // extension: the coset shift by 2717^j and the zero rows j >= rows are
-    // folded into the first pass when it is a two-sub-step kernel (it then
-    // reads coef directly); otherwise a separate kernel builds the input.
-    const bool fold = !pext->passes.empty() && pext->passes[0].v->kind == VK_TWO_STAGE;
-    const u64 *ct = fold ? coset_table(rows) : nullptr;
-    if (fold && !ct) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-    if (fold) {
+    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m, nullptr, 0, sct))) return rc;
+    if (fold_scale) {
+        if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, nullptr, rows))) return rc;
+    } else if (fold) {
         if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, ct, rows))) return rc;
     } else {
         PowTable g = power_table(2717, rows);

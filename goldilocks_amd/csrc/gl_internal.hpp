@@ -35,10 +35,12 @@ struct PassArgs {
     const u64 *tw_hi;
     u64 scale;         // multiplier applied to every output when FLAG_SCALE
     const u64 *coset;  // FLAG_COSET: input row j < coset_rows is multiplied by coset[j]
-    u64 coset_rows;    //   (Montgomery g^j); rows >= coset_rows read as zero
+    u64 coset_rows;    //   (Montgomery g^j); rows >= coset_rows read as zero (also FLAG_ZEXT)
+    const u64 *row_scale;  // FLAG_ROW_SCALE: output row r is multiplied by row_scale[r]
+    u64 SS;            //   S / S0: rows between consecutive k (row of position q is q / S0)
     u32 flags;
-    // x / ninst and x / S for x < 2^31 as (x * mul) >> sh (fastdiv_params)
-    u32 ninst_mul, ninst_sh, S_mul, S_sh;
+    // x / ninst, x / S and x / S0 for x < 2^31 as (x * mul) >> sh (fastdiv_params)
+    u32 ninst_mul, ninst_sh, S_mul, S_sh, S0_mul, S0_sh;
 };
 
 // Multiply-shift constants for dividing any x < 2^31 by d >= 1: with
@@ -75,6 +77,8 @@ enum : u32 {
     FLAG_SCALE = 4u,    // multiply outputs by `scale` (1/n for intt)
     FLAG_TWIDDLE_FULL = 8u,  // tw_lo is a full table: output k of instance m uses tw_lo[k + R*m]
     FLAG_COSET = 16u,  // first pass of the commit's extension: coset shift + zero rows
+    FLAG_ZEXT = 32u,   // first pass: rows >= coset_rows read as zero (no shift)
+    FLAG_ROW_SCALE = 64u,  // last pass: output row r times row_scale[r] (replaces FLAG_SCALE)
 };
 // All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
 // are applied with mont_mul.
@@ -112,6 +116,8 @@ void launch_rader_big_pointwise(u64 *C, u64 *x0, const u64 *spec, u64 ninst, hip
 void launch_rader_big_scatter(const PassArgs &a, const u32 *permi, const u64 *D, const u64 *X0, hipStream_t s);
 void launch_pointwise_mont(u64 *a, const u64 *b, u64 n, hipStream_t s);
 void launch_power_series(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s);
+// out[j] = in[j] * scale (Montgomery product; scale in Montgomery form), j < count.
+void launch_scale_table(u64 *out, const u64 *in, u64 count, u64 scale, hipStream_t s);
 void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blowup,
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 

@@ -30,7 +30,7 @@ __device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
 // then one product per output to advance (exact field arithmetic, so the
 // result equals the per-output lookup).
 template <int NK, int KSTEP>
-__device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 m, u32 k0, u32 R,
+__device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 s, u32 m, u32 k0, u32 R,
                                               u64 *v) {
     if (a.flags & FLAG_TWIDDLE_FULL) {
         const u64 *tw = a.tw_lo + (u64)R * (u64)m + k0;
@@ -51,6 +51,13 @@ __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 d
     if (a.flags & FLAG_SCALE) {
 #pragma unroll
         for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], a.scale);
+    } else if (a.flags & FLAG_ROW_SCALE) {
+        // output j sits at position s + S*(k0 + j*KSTEP + R*m) of its item:
+        // row s/S0 + SS*(k0 + R*m) + SS*KSTEP*j
+        const u64 *rs = a.row_scale + fastdiv(s, a.S0_mul, a.S0_sh) + a.SS * ((u64)k0 + (u64)R * m);
+        const u64 rstep = a.SS * (u64)KSTEP;
+#pragma unroll
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], rs[rstep * (u64)j]);
     }
 #pragma unroll
     for (int j = 0; j < NK; j++) dst[dstep * (u64)j] = v[j];
@@ -114,6 +121,15 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
                     const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
                     x[ra] = j < a.coset_rows ? mont_mul(p[step * (u64)ra], a.coset[j]) : 0;
                 }
+            } else if (a.flags & FLAG_ZEXT) {
+                // zero-extended column: rows j >= coset_rows read as zero
+                const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+                const u64 step = a.SMp * (u64)R2;
+#pragma unroll
+                for (int ra = 0; ra < R1; ra++) {
+                    const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+                    x[ra] = j < a.coset_rows ? p[step * (u64)ra] : 0;
+                }
             } else {
                 const u64 *p = src + (u64)i + a.SMp * (u64)rb;
                 const u64 step = a.SMp * (u64)R2;
@@ -134,7 +150,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         } else if (valid) {
             // single sub-step pass: k = ka
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * (u64)R * (u64)m;
-            store_outputs<R1, 1>(a, dst, a.S, m, 0, R, x);
+            store_outputs<R1, 1>(a, dst, a.S, s, m, 0, R, x);
         }
     }
     if constexpr (R2 > 1) {
@@ -166,7 +182,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
             dft<R2>(z);
             if (valid) {
                 u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
-                store_outputs<R2, R1>(a, dst, a.S * (u64)R1, m, (u32)ka, (u32)R, z);
+                store_outputs<R2, R1>(a, dst, a.S * (u64)R1, s, m, (u32)ka, (u32)R, z);
             }
         }
     }
@@ -288,7 +304,7 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
             locate(a, I, b, i, s, m);
             const u32 kk = (u32)(k1 + R1 * k2);
             u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)kk + (u64)R * (u64)m);
-            store_outputs<R3, R1 * R2>(a, dst, a.S * (u64)(R1 * R2), m, kk, (u32)R, z);
+            store_outputs<R3, R1 * R2>(a, dst, a.S * (u64)(R1 * R2), s, m, kk, (u32)R, z);
         }
     }
 }
@@ -447,6 +463,18 @@ void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 
 __global__ void power_series_kernel(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits) {
     for (u64 j = blockIdx.x * (u64)blockDim.x + threadIdx.x; j < count; j += (u64)gridDim.x * blockDim.x)
         out[j] = mont_mul(lo[j & ((1ull << lo_bits) - 1)], hi[j >> lo_bits]);
+}
+
+__global__ void scale_table_kernel(u64 *out, const u64 *in, u64 count, u64 scale) {
+    for (u64 j = blockIdx.x * (u64)blockDim.x + threadIdx.x; j < count; j += (u64)gridDim.x * blockDim.x)
+        out[j] = mont_mul(in[j], scale);
+}
+
+void launch_scale_table(u64 *out, const u64 *in, u64 count, u64 scale, hipStream_t s) {
+    if (count == 0) return;
+    const u64 blocks = (count + 255) / 256;
+    const unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
+    hipLaunchKernelGGL(scale_table_kernel, dim3(grid), dim3(256), 0, s, out, in, count, scale);
 }
 
 void launch_power_series(u64 *out, u64 count, const u64 *lo, const u64 *hi, u32 lo_bits, hipStream_t s) {

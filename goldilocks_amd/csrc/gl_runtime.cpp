@@ -301,6 +301,29 @@ const u64 *coset_table(u64 rows) {
     return (const u64 *)d;
 }
 
+static std::map<u64, const u64 *> g_scaled_coset_tables;
+const u64 *scaled_coset_table(u64 rows) {
+    const u64 *ct = coset_table(rows);
+    if (!ct) return nullptr;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_scaled_coset_tables.find(rows);
+    if (it != g_scaled_coset_tables.end()) return it->second;
+    void *d = nullptr;
+    if (hipMalloc(&d, rows * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    launch_scale_table((u64 *)d, ct, rows, to_mont(inv(rows % P)), nullptr);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_tw_allocs.push_back(d);
+    g_scaled_coset_tables[rows] = (const u64 *)d;
+    return (const u64 *)d;
+}
+
 // Inner twiddles of a two-sub-step pass R = R1 x R2, ordered so that one
 // thread's R1 - 1 multipliers are consecutive: entry rb*R1 + ka =
 // w_R^(ka*rb) (Montgomery form).
@@ -709,6 +732,8 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     a.Mp = (u32)Mp;
     fastdiv_params(ninst, a.ninst_mul, a.ninst_sh);
     fastdiv_params(S, a.S_mul, a.S_sh);
+    fastdiv_params(S0, a.S0_mul, a.S0_sh);
+    a.SS = S / S0;
     if (pp.v->kind == VK_RADER257) {
         a.tw_inner = rader257_table();
     } else if (pp.v->kind == VK_THREE_STAGE) {
@@ -734,12 +759,18 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         }
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
-    if (coset && p == 0) {
-        a.flags |= FLAG_COSET;
+    if ((coset || coset_rows) && p == 0) {
+        if (pp.v->kind != VK_TWO_STAGE) return set_error(GL_E_ARG, "coset input needs a two-sub-step first pass");
+        a.flags |= coset ? FLAG_COSET : FLAG_ZEXT;
         a.coset = coset;
         a.coset_rows = coset_rows;
     }
-    if (inverse && last && scale_out) {
+    if (inverse && last && row_scale) {
+        if (pp.v->kind != VK_TWO_STAGE && pp.v->kind != VK_THREE_STAGE)
+            return set_error(GL_E_ARG, "row scale needs a single-kernel last pass");
+        a.flags |= FLAG_ROW_SCALE;
+        a.row_scale = row_scale;
+    } else if (inverse && last && scale_out) {
         a.flags |= FLAG_SCALE;
         a.scale = to_mont(inv(n % P));
     }
@@ -857,13 +888,15 @@ int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count) {
 }
 
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
-             hipStream_t stream, bool scale_out, const u64 *input, const u64 *coset, u64 coset_rows) {
+             hipStream_t stream, bool scale_out, const u64 *input, const u64 *coset, u64 coset_rows,
+             const u64 *row_scale) {
     if (plan.passes.empty() || count == 0) return GL_OK;
     PassCursor c{&plan, data, tmp, S0, count, bstride, inverse};
     c.scale_out = scale_out;
     c.input = input;
     c.coset = coset;
     c.coset_rows = coset_rows;
+    c.row_scale = row_scale;
     c.begin();
     while (!c.done()) {
         const PassVariant *v = plan.passes[c.p].v;
@@ -898,6 +931,7 @@ void clear_caches() {
         g_inner_tables.clear();
         g_inner_matrices.clear();
         g_coset_tables.clear();
+        g_scaled_coset_tables.clear();
         g_full_tables.clear();
         g_rader257 = nullptr;
         g_big_rader = BigRaderTables();

@@ -94,8 +94,12 @@ struct PassCursor {
     const u64 *input = nullptr;  // first pass reads here instead of data (out of place)
     // first pass only (two-sub-step variants): input row j < coset_rows is
     // multiplied by coset[j], rows above read as zero (FLAG_COSET)
+    // (coset == nullptr with coset_rows > 0: zero rows only, FLAG_ZEXT)
     const u64 *coset = nullptr;
     u64 coset_rows = 0;
+    // inverse, last pass (two-sub-step variants): output row r is multiplied
+    // by row_scale[r] instead of 1/n (FLAG_ROW_SCALE)
+    const u64 *row_scale = nullptr;
     // evolving state
     size_t p = 0;
     u64 S = 0, M = 0;
@@ -113,9 +117,12 @@ struct PassCursor {
 // result lands in data (same layout), leaving input untouched.
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
              hipStream_t stream, bool scale_out = true, const u64 *input = nullptr, const u64 *coset = nullptr,
-             u64 coset_rows = 0);
+             u64 coset_rows = 0, const u64 *row_scale = nullptr);
 // Montgomery 2717^j for j < rows (device, cached).
 const u64 *coset_table(u64 rows);
+// Montgomery 2717^j / rows for j < rows (device, cached): the intt scale and
+// the coset shift of the commit's extension in one factor per row.
+const u64 *scaled_coset_table(u64 rows);
 // True when the plan contains a pass that is not a single kernel (the
 // radix-65537 Rader pass), which grouped batch launches cannot include.
 bool plan_is_composite(const Plan &plan);
