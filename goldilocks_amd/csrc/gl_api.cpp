@@ -51,7 +51,9 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
 
 // 21 words of per-stream device scratch for a layer's padded partial block
 // (layers on one stream run in order, so they share it).
-u64 *k12_stage(hipStream_t s) { return stream_scratch(s, 32, 6); }
+// two 32-word halves: a layer reads one staged partial block and stages the
+// next layer's in the other
+u64 *k12_stage(hipStream_t s) { return stream_scratch(s, 64, 6); }
 
 int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
                     hipStream_t s) {
@@ -62,9 +64,12 @@ int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
     uint8_t *dst = d_out;
     u64 *stage = k12_stage(s);
     if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    int cur = 0;
+    bool ready = false;
     while (len > HASH) {
-        launch_k12_layer(layer, len, dst, stage, s);
+        ready = launch_k12_layer(layer, len, dst, stage + 32 * cur, ready, stage + 32 * (1 - cur), s);
         GL_HIP(hipGetLastError());
+        cur = 1 - cur;
         layer = dst;
         len = next_layer(len);
         dst += len;
@@ -322,7 +327,7 @@ int gl_k12_layer_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, void *stre
     if (len && (!d_in || !d_out)) return set_error(GL_E_ARG, "null device pointer");
     u64 *stage = k12_stage(as_stream(stream));
     if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    launch_k12_layer(d_in, len, d_out, stage, as_stream(stream));
+    launch_k12_layer(d_in, len, d_out, stage, false, nullptr, as_stream(stream));
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -340,7 +345,7 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
     u64 *stage = k12_stage(s);
     if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    launch_k12_layer(din, len, dout, stage, s);
+    launch_k12_layer(din, len, dout, stage, false, nullptr, s);
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out, dout, olen, hipMemcpyDeviceToHost, s));
     GL_HIP(hipStreamSynchronize(s));

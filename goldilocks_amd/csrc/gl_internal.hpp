@@ -122,8 +122,14 @@ void launch_coset_extend(const u64 *coef, u64 *ext, u64 rows, u64 cols, u64 blow
                          const u64 *pw_lo, const u64 *pw_hi, u32 lo_bits, hipStream_t s);
 
 // K12 kernels (gl_k12.hip).
-// stage: 21 words of device scratch (the padded partial last block)
-void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage, hipStream_t s);
+// Hash one layer (len bytes at in) into out.  stage_in: 21 words of device
+// scratch holding this layer's padded partial last block; when stage_ready
+// is false it is built first.  stage_out (21 words, distinct from
+// stage_in, or null): the kernel also writes the next layer's padded
+// partial last block there.  Returns true when it did (the next call can
+// pass stage_out as its stage_in with stage_ready = true).
+bool launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage_in, bool stage_ready, u64 *stage_out,
+                      hipStream_t s);
 void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hipStream_t s);
 
 // Merkle tree geometry for openings: layer l has len[l] bytes at base[l]

@@ -146,7 +146,8 @@ __device__ __forceinline__ void absorb_padded(u64 *a, const uint8_t *in, u64 l) 
 // words k12_stage_tail_kernel wrote.  Kept free of byte-wise paths so the
 // state and the round temporaries fit in registers.
 __global__ __launch_bounds__(256) void k12_layer_kernel(const u64 *__restrict__ in, u64 nfull, u64 nblocks,
-                                                        const u64 *__restrict__ stage, u64 *__restrict__ out) {
+                                                        const u64 *__restrict__ stage, u64 *__restrict__ out,
+                                                        u64 *__restrict__ stage_next, u64 next_first) {
     for (u64 blk = blockIdx.x * (u64)blockDim.x + threadIdx.x; blk < nblocks;
          blk += (u64)gridDim.x * blockDim.x) {
         u64 a[25];
@@ -167,6 +168,22 @@ __global__ __launch_bounds__(256) void k12_layer_kernel(const u64 *__restrict__ 
         o[1] = a[1];
         o[2] = a[2];
         o[3] = a[3];
+        // Outputs of blocks >= next_first form the next layer's partial
+        // block (32 q bytes, q = nblocks - next_first in 1..4): copy them
+        // into its padded form; the last block adds the padding words
+        // (0x00, 0x07 at bytes 32q, 32q + 1; 0x80 at byte 167).
+        if (stage_next && blk >= next_first) {
+            u64 *t = stage_next + 4 * (blk - next_first);
+            t[0] = a[0];
+            t[1] = a[1];
+            t[2] = a[2];
+            t[3] = a[3];
+            if (blk + 1 == nblocks) {
+                const u64 q4 = 4 * (nblocks - next_first);
+                for (u64 w = q4; w < 21; w++)
+                    stage_next[w] = (w == q4 ? 0x0700ull : 0) | (w == 20 ? 0x8000000000000000ull : 0);
+            }
+        }
     }
 }
 
@@ -207,23 +224,29 @@ __global__ __launch_bounds__(64) void k12_layer_bytes_kernel(const uint8_t *__re
     }
 }
 
-void launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage, hipStream_t s) {
+bool launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage_in, bool stage_ready, u64 *stage_out,
+                      hipStream_t s) {
     const u64 nblocks = (len + 159) / 160;
-    if (nblocks == 0) return;
+    if (nblocks == 0) return false;
     const bool aligned = ((uintptr_t)in % 8) == 0 && ((uintptr_t)out % 8) == 0;
     if (!aligned) {  // every block byte by byte
         u64 g = (nblocks + 63) / 64;
         unsigned grid = (unsigned)(g < 65536 ? g : 65536);
         hipLaunchKernelGGL(k12_layer_bytes_kernel, dim3(grid), dim3(64), 0, s, in, len, out, (u64)0, nblocks);
-        return;
+        return false;
     }
     const u64 nfull = len / 160;
-    if (nfull < nblocks)
-        hipLaunchKernelGGL(k12_stage_tail_kernel, dim3(1), dim3(64), 0, s, in, len, nfull, stage);
+    if (nfull < nblocks && !stage_ready)
+        hipLaunchKernelGGL(k12_stage_tail_kernel, dim3(1), dim3(64), 0, s, in, len, nfull, stage_in);
+    // the next layer (32 bytes per block) has a partial last block to stage
+    const u64 next_len = nblocks * 32;
+    const bool produce = stage_out && next_len > 32 && next_len % 160 != 0;
+    const u64 next_first = produce ? (next_len / 160) * 5 : nblocks;
     u64 g = (nblocks + 255) / 256;
     unsigned grid = (unsigned)(g < 65536 ? g : 65536);
     hipLaunchKernelGGL(k12_layer_kernel, dim3(grid), dim3(256), 0, s, (const u64 *)in, nfull, nblocks,
-                       (const u64 *)stage, (u64 *)out);
+                       (const u64 *)stage_in, (u64 *)out, produce ? stage_out : nullptr, next_first);
+    return produce;
 }
 
 // Proof-of-work scan (pcs/src/pow.rs:32-60): nonces [start, start+count) are
