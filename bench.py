@@ -159,6 +159,20 @@ def bench_ntt(args, torch, gl):
         host_s = statistics.median(host_times)
         host = {"gelem_s": round(n / host_s / 1e9, 3), "ms": round(host_s * 1e3, 3),
                 "path": "gl_ntt on a pageable host array (PCIe copies included)"}
+    # Achievable streaming rate on this device: a plain device copy of the
+    # same n words (8n bytes read, 8n written), timed like the steps.
+    buf = torch.empty_like(x)
+    for _ in range(3):
+        buf.copy_(x)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    c0.record(stream)
+    for _ in range(20):
+        buf.copy_(x)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 16.0 * n / (c0.elapsed_time(c1) / 20 * 1e-3) / 1e9
+    del buf
     plan = gl.plan_info(n)
     # Dominant kernel: largest share of the timed region.  Each NTT pass reads
     # and writes every word once, so a launch moves 16 n algorithmic bytes.
@@ -189,7 +203,8 @@ def bench_ntt(args, torch, gl):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_for(kname, f"ntt_2^{args.log_n}"),
-                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": 16 * n},
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": 16 * n,
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
     }
 
 
