@@ -181,7 +181,7 @@ def bench_ntt(args, torch, gl):
     achieved = 16.0 * n / (avg_ms * 1e-3) / 1e9
     gelem = n / (ms * 1e-3) / 1e9
     return {
-        "metric": "Gelem/s, 2^24-point forward Goldilocks NTT (HBM-resident)",
+        "metric": f"Gelem/s, 2^{args.log_n}-point forward Goldilocks NTT (HBM-resident)",
         "value": round(gelem, 3),
         "unit": "Gelem/s",
         "n_gpus": 1,
