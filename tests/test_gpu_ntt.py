@@ -154,24 +154,31 @@ def test_large_device(gl, oracle, k):
         assert np.array_equal(host_y, oracle.ntt(host_x))
 
 
-def _sum_mod_p(t):
-    """Exact sum of u64 words (an int64 GPU tensor, < 2^31 elements) mod p:
-    32-bit halves summed in int64 cannot overflow."""
+def _sum_mod_p(t, chunk=1 << 28):
+    """Exact sum of u64 words (a 1-D int64 GPU tensor, any length) mod p:
+    32-bit halves of at most 2^28 words are summed in int64 (no overflow),
+    chunk sums are added as Python integers."""
     import torch
-    lo = (t & 0xFFFFFFFF).sum().item()
-    hi = torch.bitwise_right_shift(t, 32).bitwise_and(0xFFFFFFFF).sum().item()
-    return (hi * (1 << 32) + lo) % P
+    total = 0
+    for c in range(0, t.numel(), chunk):
+        u = t[c:c + chunk]
+        lo = (u & 0xFFFFFFFF).sum().item()
+        hi = torch.bitwise_right_shift(u, 32).bitwise_and(0xFFFFFFFF).sum().item()
+        total += hi * (1 << 32) + lo
+    return total % P
 
 
-@pytest.mark.parametrize("k", [30, 31])
+@pytest.mark.parametrize("k", [30, 31, 32])
 def test_max_size_device(gl, k):
     """Lengths where word positions pass 2^30 / 2^31 and byte offsets pass
-    2^33 / 2^34 (8 and 16 GiB): X[0] = sum x_j and X[n/2] = sum (-1)^j x_j
-    exactly, a run is deterministic, and the roundtrip is exact."""
+    2^33 / 2^34 (8 and 16 GiB), up to 2^32, the largest power of two that
+    divides p - 1 (32 GiB, four passes): X[0] = sum x_j and
+    X[n/2] = sum (-1)^j x_j exactly, a run is deterministic, and the
+    roundtrip is exact."""
     import torch
     n = 1 << k
     free, _ = torch.cuda.mem_get_info()
-    if free < 4 * 8 * n:
+    if free < 4 * 8 * n + (4 << 30):
         pytest.skip(f"needs {4 * 8 * n >> 30} GiB of device memory")
     note(f"max size n=2^{k}")
     x = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -180,7 +187,7 @@ def test_max_size_device(gl, k):
     gl.ntt(y)
     torch.cuda.synchronize()
     assert int(y[0].item()) % (1 << 64) == _sum_mod_p(x)
-    even, odd = _sum_mod_p(x[0::2].contiguous()), _sum_mod_p(x[1::2].contiguous())
+    even, odd = _sum_mod_p(x[0::2]), _sum_mod_p(x[1::2])
     assert int(y[n // 2].item()) % (1 << 64) == (even - odd) % P
     z = x.clone()
     gl.ntt(z)
