@@ -133,6 +133,15 @@ def bench_ntt(args, torch, gl):
     per_sorted = sorted(per)
     p50 = per_sorted[len(per) // 2]
     p99 = per_sorted[min(len(per) - 1, int(0.99 * len(per)))]
+    # The inverse transform on the same words, timed the same way (reported
+    # beside the forward value).
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        gl.intt(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    inv_ms = e0.elapsed_time(e1) / args.steps
     # Kernel durations: the same steps again with every pass launch bracketed
     # by HIP events on the launch stream (kept out of the region above so the
     # extra event records do not perturb the step time).
@@ -197,6 +206,7 @@ def bench_ntt(args, torch, gl):
         "config": {"workload": f"forward NTT, n=2^{args.log_n}, in place, words resident in HBM",
                    "n": n, "passes": [p["kernel"] for p in plan]},
         "latency_ms": {"p50": round(p50, 5), "p99": round(p99, 5)},
+        "inverse": {"ms_per_step": round(inv_ms, 5), "gelem_s": round(n / (inv_ms * 1e-3) / 1e9, 3)},
         "transform_hbm_frac": round(16.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
         "host_buffer": host,
