@@ -99,10 +99,13 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
             for (int ka = 1; ka < R1; ka++) w[ka] = twr[ka];
         }
+        // lanes past the last instance load the last instance's words (their
+        // results are never stored), so the loads need no branch and no
+        // zero fill
         u64 x[R1];
-        u32 b = 0, i = 0, s = 0, m = 0;
-        if (valid) {
-            locate(a, I, b, i, s, m);
+        u32 b, i, s, m;
+        {
+            locate(a, valid ? I : a.total_inst - 1, b, i, s, m);
             const u64 *src = a.in + (u64)b * a.bstride;
             if (a.flags & FLAG_REVERSE) {
 #pragma unroll
@@ -136,9 +139,6 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
                 for (int ra = 0; ra < R1; ra++) x[ra] = p[step * (u64)ra];
             }
-        } else {
-#pragma unroll
-            for (int ra = 0; ra < R1; ra++) x[ra] = 0;
         }
         dft<R1>(x);
         if constexpr (R2 > 1) {
@@ -232,9 +232,10 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
 #pragma unroll
         for (int k1 = 1; k1 < R1; k1++) w[k1] = a.tw_inner[(k1 * c) % R];
         u64 x[R1];
-        if (I < a.total_inst) {
+        {
+            // lanes past the last instance load its words; never stored
             u32 b, i, s, m;
-            locate(a, I, b, i, s, m);
+            locate(a, I < a.total_inst ? I : a.total_inst - 1, b, i, s, m);
             const u64 *src = a.in + (u64)b * a.bstride;
             if (a.flags & FLAG_REVERSE) {
 #pragma unroll
@@ -247,9 +248,6 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
 #pragma unroll
                 for (int r1 = 0; r1 < R1; r1++) x[r1] = p[a.SMp * (u64)(r1 * R23)];
             }
-        } else {
-#pragma unroll
-            for (int r1 = 0; r1 < R1; r1++) x[r1] = 0;
         }
         dft<R1>(x);
 #pragma unroll
