@@ -12,8 +12,9 @@ def mixed_sizes(count, seed, lo=10, hi=20, factors=(1, 3, 5, 15, 17, 257)):
 def bench_batch(args, torch, gl):
     """1024 forward transforms, n = 2^k * f (k uniform in [lo, hi], default
     [10, 20], f from {1, 3, 5, 15, 17, 257}), device-resident: grouped
-    launches vs one launch sequence per transform; per-batch latency p50/p99
-    over the repetitions."""
+    launches (the reported value), one launch sequence per transform, and the
+    batch captured once as a hipGraph and replayed; per-batch latency
+    p50/p99 over the repetitions."""
     lo, hi = args.batch_k
     sizes = mixed_sizes(1024, 7, lo, hi)
     total = sum(sizes)
@@ -27,22 +28,29 @@ def bench_batch(args, torch, gl):
 
     res = {}
     reps = max(3, min(args.steps, 12))
-    for batched in (True, False):
-        run(batched)
+
+    def timed(key, fn):
+        fn()
         torch.cuda.synchronize()
         times = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            run(batched)
+            fn()
             e1.record(stream)
             torch.cuda.synchronize()
             times.append(e0.elapsed_time(e1))
         times.sort()
         med = times[len(times) // 2]
-        res["batched" if batched else "unbatched"] = {
-            "ms_p50": round(med, 3), "ms_p99": round(times[min(len(times) - 1, int(0.99 * len(times)))], 3),
-            "gelem_s": round(total / (med * 1e-3) / 1e9, 3)}
+        res[key] = {"ms_p50": round(med, 3), "ms_p99": round(times[min(len(times) - 1, int(0.99 * len(times)))], 3),
+                    "gelem_s": round(total / (med * 1e-3) / 1e9, 3)}
+
+    timed("batched", lambda: run(True))
+    timed("unbatched", lambda: run(False))
+    # the same batch planned once and replayed as one captured hipGraph
+    plan = gl.BatchPlan(bufs)
+    timed("graph", lambda: plan.launch(stream))
+    plan.close()
     return {
         "metric": "Gelem/s, mixed-size batched NTT (1024 transforms); p50/p99 batch latency",
         "value": res["batched"]["gelem_s"],
@@ -58,7 +66,9 @@ def bench_batch(args, torch, gl):
         "latency_ms": {"p50": res["batched"]["ms_p50"], "p99": res["batched"]["ms_p99"]},
         "batched": res["batched"],
         "unbatched": res["unbatched"],
+        "graph": res["graph"],
         "speedup_vs_unbatched": round(res["unbatched"]["ms_p50"] / res["batched"]["ms_p50"], 3),
+        "graph_speedup_vs_unbatched": round(res["unbatched"]["ms_p50"] / res["graph"]["ms_p50"], 3),
     }
 
 
