@@ -44,14 +44,16 @@ struct Schedule {
     size_t meta_bytes() const { return desc_bytes() + starts.size() * sizeof(u32); }
 };
 
-// Transforms are grouped in chunks of at most this many words, so that a
-// chunk's data and scratch stay resident in the device's last-level cache
-// between its passes; a transform larger than a chunk runs by itself, where
-// its own passes get that reuse.  GOLDILOCKS_BATCH_CHUNK_BYTES overrides.
+// Transforms are grouped in chunks of at most this many bytes; a transform
+// larger than a chunk runs by itself.  Bigger chunks mean fewer, fuller
+// grouped launches, smaller ones keep a chunk's words in the last-level
+// cache between its passes.  1 GiB measured best on MI355X for both the
+// 2^4..2^12 and the 2^10..2^20 mixed batches (64 MiB, 256 MiB, 512 MiB and
+// 4 GiB were slower).  GOLDILOCKS_BATCH_CHUNK_BYTES overrides.
 std::atomic<u64> &chunk_bytes_ref() {
     static std::atomic<u64> b{[] {
         const char *e = getenv("GOLDILOCKS_BATCH_CHUNK_BYTES");
-        return e ? (u64)strtoull(e, nullptr, 10) : (u64)64 << 20;
+        return e ? (u64)strtoull(e, nullptr, 10) : (u64)1 << 30;
     }()};
     return b;
 }

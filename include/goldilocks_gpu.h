@@ -83,10 +83,9 @@ int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int invers
 
 /* Mixed-size batch on device buffers d[i] of n[i] words.  Transforms are
  * packed in order into chunks of at most gl_set_batch_chunk_bytes bytes
- * (default 64 MiB, or GOLDILOCKS_BATCH_CHUNK_BYTES); each chunk runs as
- * grouped launches, one per (pass level, kernel variant), so its words stay
- * cache resident between passes.  Larger transforms and radix-65537 ones run
- * on their own.  Returns after the stream has consumed the launch data. */
+ * (default 1 GiB, or GOLDILOCKS_BATCH_CHUNK_BYTES); each chunk runs as
+ * grouped launches, one per (pass level, kernel variant).  Larger transforms
+ * and radix-65537 ones run on their own.  Returns after the stream has consumed the launch data. */
 int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, int inverse, void *stream);
 
 /* Chunk size of the grouped batch path; returns the previous value.  Results
