@@ -17,6 +17,20 @@
 
 namespace gl {
 
+// Byte offset k * stride (stride a uniform word count, k a compile-time
+// constant) formed on the scalar unit; the empty asm keeps the product out
+// of a per-element v_mad_u64_u32, so each strided access costs one 64-bit
+// vector add.
+__device__ __forceinline__ u64 uniform_offset(u64 stride, int k) {
+    u64 o = stride * (u64)k * 8;
+    asm("" : "+s"(o));
+    return o;
+}
+template <typename W>
+__device__ __forceinline__ W *at_offset(W *p, u64 bytes) {
+    return (W *)((char *)p + bytes);
+}
+
 // Montgomery form of w_M^e from the two-level power table.
 __device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
     return mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]);
@@ -60,7 +74,7 @@ __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 d
         for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], rs[rstep * (u64)j]);
     }
 #pragma unroll
-    for (int j = 0; j < NK; j++) dst[dstep * (u64)j] = v[j];
+    for (int j = 0; j < NK; j++) *at_offset(dst, uniform_offset(dstep, j)) = v[j];
 }
 
 // LDS tile of the two-sub-step pass: instance t owns TS words, row ka of it
@@ -122,7 +136,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
                 for (int ra = 0; ra < R1; ra++) {
                     const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
-                    x[ra] = j < a.coset_rows ? mont_mul(p[step * (u64)ra], a.coset[j]) : 0;
+                    x[ra] = j < a.coset_rows ? mont_mul(*at_offset(p, uniform_offset(step, ra)), a.coset[j]) : 0;
                 }
             } else if (a.flags & FLAG_ZEXT) {
                 // zero-extended column: rows j >= coset_rows read as zero
@@ -131,13 +145,13 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
 #pragma unroll
                 for (int ra = 0; ra < R1; ra++) {
                     const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
-                    x[ra] = j < a.coset_rows ? p[step * (u64)ra] : 0;
+                    x[ra] = j < a.coset_rows ? *at_offset(p, uniform_offset(step, ra)) : 0;
                 }
             } else {
                 const u64 *p = src + (u64)i + a.SMp * (u64)rb;
                 const u64 step = a.SMp * (u64)R2;
 #pragma unroll
-                for (int ra = 0; ra < R1; ra++) x[ra] = p[step * (u64)ra];
+                for (int ra = 0; ra < R1; ra++) x[ra] = *at_offset(p, uniform_offset(step, ra));
             }
         }
         dft<R1>(x);
