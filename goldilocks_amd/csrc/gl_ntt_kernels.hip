@@ -43,25 +43,18 @@ __device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
 // A * B^j with A = w^(m*k0), B = w^(m*kstep): two table lookups per thread,
 // then one product per output to advance (exact field arithmetic, so the
 // result equals the per-output lookup).
+template <int NK>
+__device__ __forceinline__ void store_all(u64 *dst, u64 dstep, const u64 *v) {
+#pragma unroll
+    for (int j = 0; j < NK; j++) *at_offset(dst, uniform_offset(dstep, j)) = v[j];
+}
+
 template <int NK, int KSTEP>
 __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 s, u32 m, u32 k0, u32 R,
                                               u64 *v) {
-    if (a.flags & FLAG_TWIDDLE_FULL) {
-        const u64 *tw = a.tw_lo + (u64)R * (u64)m + k0;
-        u64 w[NK];
-#pragma unroll
-        for (int j = 0; j < NK; j++) w[j] = tw[j * KSTEP];
-#pragma unroll
-        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], w[j]);
-    } else if (a.flags & FLAG_TWIDDLE) {
-        u64 w = tw_pow(a, (u64)m * k0);
-        const u64 B = tw_pow(a, (u64)m * KSTEP);
-#pragma unroll
-        for (int j = 0; j < NK; j++) {
-            v[j] = mont_mul(v[j], w);
-            if (j + 1 < NK) w = mont_mul(w, B);
-        }
-    }
+    // The scale factors commute with the twiddle (exact products), so they
+    // come first and each twiddle mode ends in its own store loop: no join
+    // of the branches' registers before the stores.
     if (a.flags & FLAG_SCALE) {
 #pragma unroll
         for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], a.scale);
@@ -71,10 +64,28 @@ __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 d
         const u64 *rs = a.row_scale + fastdiv(s, a.S0_mul, a.S0_sh) + a.SS * ((u64)k0 + (u64)R * m);
         const u64 rstep = a.SS * (u64)KSTEP;
 #pragma unroll
-        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], rs[rstep * (u64)j]);
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], *at_offset(rs, uniform_offset(rstep, j)));
     }
+    if (a.flags & FLAG_TWIDDLE_FULL) {
+        const u64 *tw = a.tw_lo + (u64)R * (u64)m + k0;
+        u64 w[NK];
 #pragma unroll
-    for (int j = 0; j < NK; j++) *at_offset(dst, uniform_offset(dstep, j)) = v[j];
+        for (int j = 0; j < NK; j++) w[j] = tw[j * KSTEP];
+#pragma unroll
+        for (int j = 0; j < NK; j++) v[j] = mont_mul(v[j], w[j]);
+        store_all<NK>(dst, dstep, v);
+    } else if (a.flags & FLAG_TWIDDLE) {
+        u64 w = tw_pow(a, (u64)m * k0);
+        const u64 B = tw_pow(a, (u64)m * KSTEP);
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+            v[j] = mont_mul(v[j], w);
+            if (j + 1 < NK) w = mont_mul(w, B);
+        }
+        store_all<NK>(dst, dstep, v);
+    } else {
+        store_all<NK>(dst, dstep, v);
+    }
 }
 
 // LDS tile of the two-sub-step pass: instance t owns TS words, row ka of it
