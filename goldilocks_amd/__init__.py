@@ -78,6 +78,8 @@ def lib():
             "gl_variants": (sz, [vp, vp, vp, sz]),
             "gl_plan_set": (i32, [u64, vp, sz]),
             "gl_plan_variants": (i32, [u64, vp, sz, psz]),
+            "gl_plan_variants_cols": (i32, [u64, vp, sz, psz]),
+            "gl_plan_set_cols": (i32, [u64, vp, sz]),
             "gl_variant_name": (ctypes.c_char_p, [ctypes.c_uint32]),
             "gl_set_twiddle_full_max": (u64, [u64]),
             "gl_timing_enable": (None, [i32]),
@@ -277,11 +279,21 @@ def set_twiddle_full_max(words):
     return lib().gl_set_twiddle_full_max(int(words))
 
 
-def plan_set(n, variant_ids):
-    """Install an explicit schedule for length n (empty list: default)."""
+def plan_set(n, variant_ids, cols=False):
+    """Install an explicit schedule for length n (empty list: default);
+    cols=True sets the schedule of column transforms of length n."""
     k = len(variant_ids)
     ids = (ctypes.c_uint32 * max(k, 1))(*variant_ids)
-    _check(lib().gl_plan_set(n, ids, k))
+    _check((lib().gl_plan_set_cols if cols else lib().gl_plan_set)(n, ids, k))
+
+
+def plan_ids(n, cols=False):
+    """Variant ids of the schedule for length n (column transforms: cols=True)."""
+    cap = 8
+    ids = (ctypes.c_uint32 * cap)()
+    npass = ctypes.c_size_t(0)
+    _check((lib().gl_plan_variants_cols if cols else lib().gl_plan_variants)(n, ids, cap, ctypes.byref(npass)))
+    return [ids[i] for i in range(npass.value)]
 
 
 def ntt_cols(matrix, inverse=False):

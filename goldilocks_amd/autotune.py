@@ -8,6 +8,7 @@ records it in a JSON cache (GOLDILOCKS_TUNE_CACHE or
 ~/.cache/goldilocks_amd/tune.json) that load_cache() re-installs.
 
 Usage: python -m goldilocks_amd.autotune 20 24 27   (log2 lengths)
+       python -m goldilocks_amd.autotune --cols 64 20 21 --write   (column transforms)
 """
 import itertools
 import json
@@ -47,53 +48,61 @@ def candidate_plans(n, max_extra_passes=1, min_radix=16):
     return out
 
 
-def time_plan(torch, x, n, ids, reps=20):
+def time_plan(torch, x, n, ids, reps=20, cols=0):
+    """Time the schedule `ids` (None: the installed one) on x: a length-n
+    vector, or with cols > 0 an (n, cols) matrix transformed by columns."""
     if ids is not None:
-        gl.plan_set(n, ids)
+        gl.plan_set(n, ids, cols=cols > 0)
+    run = (lambda t: gl.ntt_cols(t)) if cols else (lambda t: gl.ntt(t))
     y = x.clone()
-    gl.ntt(y)  # warm (and plan install check)
+    run(y)  # warm (and plan install check)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(reps):
-        gl.ntt(y)
+        run(y)
     e1.record(stream)
     torch.cuda.synchronize()
     z = x.clone()
-    gl.ntt(z)
+    run(z)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps, z
 
 
-def tune(n, verbose=True, max_candidates=400):
+def tune(n, verbose=True, max_candidates=400, cols=0):
+    """Time every candidate schedule for length n (contiguous transforms, or
+    with cols > 0 the columns of an (n, cols) matrix) and install the fastest."""
     import torch
-    x = torch.empty(n, dtype=torch.int64, device="cuda")
+    x = torch.empty((n, cols) if cols else n, dtype=torch.int64, device="cuda")
     gl.fill_random(x, 12345)
-    gl.plan_set(n, [])
-    reps = 20 if n <= (1 << 22) else 5
-    base_ms, ref = time_plan(torch, x, n, None, reps)
-    default = [p["kernel"] for p in gl.plan_info(n)]
+    gl.plan_set(n, [], cols=cols > 0)
+    words = n * max(cols, 1)
+    reps = 20 if words <= (1 << 22) else 5
+    base_ms, ref = time_plan(torch, x, n, None, reps, cols)
+    vs_all = {v["id"]: v for v in gl.variants()}
+    default = [vs_all[i]["name"] for i in gl.plan_ids(n, cols=cols > 0)]
     results = [(base_ms, None)]
     cands = candidate_plans(n)[:max_candidates]
     for c, ids in enumerate(cands):
-        ms, z = time_plan(torch, x, n, ids, reps)
+        ms, z = time_plan(torch, x, n, ids, reps, cols)
         if not torch.equal(z, ref):
             raise RuntimeError(f"schedule {ids} for n={n} is not bit-exact")
         results.append((ms, ids))
         if verbose and (c + 1) % 100 == 0:
             print(f"  n={n}: {c + 1}/{len(cands)} candidates timed", flush=True)
-    gl.plan_set(n, [])
+    gl.plan_set(n, [], cols=cols > 0)
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]
     if best is not None:
-        gl.plan_set(n, best)
+        gl.plan_set(n, best, cols=cols > 0)
     if verbose:
         vs = {v["id"]: v for v in gl.variants()}
         def desc(ids):
             return "default " + str(default) if ids is None else \
                 " + ".join(tune_key(vs[i]) for i in ids)
-        print(f"n={n}: {len(cands)} candidates; default {base_ms * 1e3:.1f} us", flush=True)
+        what = f"n={n}" + (f" x {cols} columns" if cols else "")
+        print(f"{what}: {len(cands)} candidates; default {base_ms * 1e3:.1f} us", flush=True)
         for ms, ids in results[:8]:
             print(f"   {ms * 1e3:9.1f} us  {desc(ids)}", flush=True)
     return best, best_ms, base_ms
@@ -109,26 +118,32 @@ def tune_key(v):
 
 def write_tune_file(entries, path=TUNE_FILE):
     """Write schedules in the text format the library loads at init:
-    "n R1xR2/T R1xR2/T ..." per line (merged with existing lines).  R2 is the
-    variant's R2 field (R2*R3 for three-stage kernels); (R1, R2, T) names one
-    compiled variant."""
+    "n R1xR2/T R1xR2/T ..." per line, "cn ..." for column transforms
+    (merged with existing lines).  R2 is the variant's R2 field (R2*R3 for
+    three-stage kernels); (R1, R2, T) names one compiled variant.  entries
+    maps n (or the string "c<n>") to a list of variant ids (None: remove)."""
     vs = {v["id"]: v for v in gl.variants()}
     keys = [tune_key(v) for v in vs.values()]
     assert len(keys) == len(set(keys)), "tune-file keys must name one variant each"
+
+    def order(key):
+        return (key.startswith("c"), int(key.lstrip("c")))
+
     lines = {}
     if os.path.exists(path):
         for line in open(path):
             if line.strip() and not line.startswith("#"):
-                lines[int(line.split()[0])] = line.strip()
+                lines[line.split()[0]] = line.strip()
     for n, ids in entries.items():
+        key = str(n)
         if ids:
-            lines[int(n)] = f"{n} " + " ".join(tune_key(vs[i]) for i in ids)
+            lines[key] = f"{key} " + " ".join(tune_key(vs[i]) for i in ids)
         else:
-            lines.pop(int(n), None)
+            lines.pop(key, None)
     with open(path, "w") as f:
         f.write("# autotuned NTT schedules for gfx950 (python -m goldilocks_amd.autotune --write ...)\n")
-        for n in sorted(lines):
-            f.write(lines[n] + "\n")
+        for key in sorted(lines, key=order):
+            f.write(lines[key] + "\n")
 
 
 def save(entries):
@@ -164,15 +179,20 @@ if __name__ == "__main__":
     if "--out" in args:
         out = args[args.index("--out") + 1]
         args = [a for a in args if a not in ("--out", out)]
+    cols = 0
+    if "--cols" in args:  # tune the column transforms of (n, cols) matrices
+        cols = int(args[args.index("--cols") + 1])
+        args = [a for i, a in enumerate(args) if a != "--cols" and (i == 0 or args[i - 1] != "--cols")]
     args = [a for a in args if a != "--write"]
     gl.lib().gl_init(0)
-    found = {}
     for arg in args:
         n = int(arg[1:]) if arg.startswith("=") else 1 << int(arg)
-        best, best_ms, base_ms = tune(n)
+        best, best_ms, base_ms = tune(n, cols=cols)
         # keep the default when the win is within timing noise
-        found[n] = best if best is not None and best_ms < 0.97 * base_ms else None
+        ids = best if best is not None and best_ms < 0.97 * base_ms else None
+        key = f"c{n}" if cols else n
         # write after every length so a cut-off run keeps what it measured
-        save({n: found[n]})
+        if not cols:
+            save({n: ids})
         if write or out:
-            write_tune_file({n: found[n]}, out or TUNE_FILE)
+            write_tune_file({key: ids}, out or TUNE_FILE)

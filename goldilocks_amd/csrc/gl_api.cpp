@@ -254,10 +254,23 @@ const char *gl_variant_name(uint32_t id) {
     return names.emplace(id, buf).first->second.c_str();
 }
 
+static int plan_variant_ids(const Plan *plan, uint32_t *ids, size_t cap, size_t *npasses);
+
 int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
     const Plan *plan = nullptr;
     int rc = get_plan(n, &plan);
     if (rc) return rc;
+    return plan_variant_ids(plan, ids, cap, npasses);
+}
+
+int gl_plan_variants_cols(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
+    const Plan *plan = nullptr;
+    int rc = get_plan_cols(n, &plan);
+    if (rc) return rc;
+    return plan_variant_ids(plan, ids, cap, npasses);
+}
+
+static int plan_variant_ids(const Plan *plan, uint32_t *ids, size_t cap, size_t *npasses) {
     if (npasses) *npasses = plan->passes.size();
     size_t cnt = 0;
     (void)variant_by_id(0, &cnt);
@@ -284,6 +297,11 @@ int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *coun
 int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants) {
     if (nvariants && !variant_ids) return set_error(GL_E_ARG, "null pointer");
     return set_plan(n, variant_ids, nvariants);
+}
+
+int gl_plan_set_cols(uint64_t n, const uint32_t *variant_ids, size_t nvariants) {
+    if (nvariants && !variant_ids) return set_error(GL_E_ARG, "null pointer");
+    return set_plan(n, variant_ids, nvariants, true);
 }
 
 int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses) {

@@ -631,11 +631,12 @@ int get_plan_cols(u64 n, const Plan **out) {
     return GL_OK;
 }
 
-int set_plan(u64 n, const u32 *ids, size_t nids) {
+int set_plan(u64 n, const u32 *ids, size_t nids, bool cols) {
     if (n == 0 || (P - 1) % n != 0) return set_error(GL_E_SIZE, "plan_set: size does not divide p - 1");
     std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto &plans = cols ? g_plans_cols : g_plans;
     if (nids == 0) {
-        g_plans.erase(n);
+        plans.erase(n);
         return GL_OK;
     }
     int cnt = 0;
@@ -653,13 +654,15 @@ int set_plan(u64 n, const u32 *ids, size_t nids) {
         prod *= R;
     }
     if (prod != n) return set_error(GL_E_ARG, "plan_set: radices do not multiply to n");
-    g_plans[n] = std::move(p);
+    plans[n] = std::move(p);
     return GL_OK;
 }
 
 // Autotuned schedules: text lines "n R1xR2/T R1xR2/T ..." (variant radices
-// and instances per workgroup, as gl_variants lists them).  Read from
-// GOLDILOCKS_TUNE_FILE, else tune_gfx950.txt next to this library.
+// and instances per workgroup, as gl_variants lists them); "cn ..." lines
+// hold the schedule of column transforms of length n (matrices of several
+// columns).  Read from GOLDILOCKS_TUNE_FILE, else tune_gfx950.txt next to
+// this library.
 static void load_tune_file() {
     std::string path;
     if (const char *e = getenv("GOLDILOCKS_TUNE_FILE")) {
@@ -682,7 +685,8 @@ static void load_tune_file() {
         char *save = nullptr;
         char *tok = strtok_r(line, " \t\r\n", &save);
         if (!tok) continue;
-        u64 n = strtoull(tok, nullptr, 10);
+        const bool cols = tok[0] == 'c';
+        u64 n = strtoull(tok + (cols ? 1 : 0), nullptr, 10);
         std::vector<u32> ids;
         bool ok = true;
         while ((tok = strtok_r(nullptr, " \t\r\n", &save))) {
@@ -701,7 +705,7 @@ static void load_tune_file() {
             }
             ids.push_back((u32)id);
         }
-        if (ok && !ids.empty()) (void)set_plan(n, ids.data(), ids.size());
+        if (ok && !ids.empty()) (void)set_plan(n, ids.data(), ids.size(), cols);
     }
     fclose(f);
 }

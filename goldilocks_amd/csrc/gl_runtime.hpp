@@ -80,7 +80,8 @@ int get_plan_cols(u64 n, const Plan **out);
 // Install an explicit pass list (variant indices as listed by gl_variants;
 // index == variant count means the radix-257 Rader pass); empty restores
 // the planner's default.
-int set_plan(u64 n, const u32 *ids, size_t nids);
+// cols: the schedule of column transforms (S0 > 1) instead of contiguous ones
+int set_plan(u64 n, const u32 *ids, size_t nids, bool cols = false);
 
 // Pass p of a plan as launch arguments.  `src_is_data` tracks the ping-pong
 // state: pass 0 reads data; non-final passes alternate data <-> tmp; the

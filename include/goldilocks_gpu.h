@@ -116,6 +116,11 @@ size_t gl_variants(uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap);
  * the planner's default. */
 int gl_plan_set(uint64_t n, const uint32_t *variant_ids, size_t nvariants);
 
+/* The same for column transforms of length n (gl_ntt_cols_dev with cols > 1,
+ * and the commit's column transforms), which keep a schedule of their own.
+ * Tuning aid, no reference counterpart. */
+int gl_plan_set_cols(uint64_t n, const uint32_t *variant_ids, size_t nvariants);
+
 /* Kernel name of variant `id` (ids as for gl_plan_set; count = radix-257
  * Rader kernel, count + 1 = radix-65537 Rader composite); NULL if unknown.
  * Tuning/introspection aid, no reference counterpart. */
@@ -124,6 +129,8 @@ const char *gl_variant_name(uint32_t id);
 /* Variant ids of the passes of the schedule for length n (the same plan
  * gl_ntt would run). */
 int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
+/* The same for the column-transform schedule of length n. */
+int gl_plan_variants_cols(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
 
 /* Output twiddles of a pass on sub-problems of length M use one M-word table
  * (one load and one product per output) when M <= words, else a two-level

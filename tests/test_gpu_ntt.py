@@ -222,3 +222,28 @@ def test_pass_timing_records_every_pass(gl):
     assert [name for name, _ in recs] == plan * 3
     assert all(ms > 0 for _, ms in recs)
     assert gl.timing_collect() == []
+
+
+def test_column_schedule_override(gl, oracle):
+    """gl_plan_set_cols installs a column-transform schedule of its own (here
+    a single-sub-step first pass, as the tuned commit schedules use); the
+    columns stay bit-exact and the contiguous schedule is untouched."""
+    import torch
+    n, cols = 4096, 6
+    vs = {v["name"]: v["id"] for v in gl.variants()}
+    ids = [vs["ntt_pass_kernel<16, 1, 256, 256>"], vs["ntt_pass_kernel<16, 16, 256, 16>"]]
+    before = gl.plan_ids(n)
+    gl.plan_set(n, ids, cols=True)
+    try:
+        assert gl.plan_ids(n, cols=True) == ids
+        assert gl.plan_ids(n) == before
+        m = oracle.fill_random(n * cols, 17).reshape(n, cols)
+        t = to_torch(m.reshape(-1)).reshape(n, cols)
+        note(f"column schedule override n={n} cols={cols}")
+        gl.ntt_cols(t)
+        torch.cuda.synchronize()
+        got = from_torch(t.reshape(-1)).reshape(n, cols)
+        for c in range(cols):
+            assert np.array_equal(got[:, c], oracle.ntt(np.ascontiguousarray(m[:, c])))
+    finally:
+        gl.plan_set(n, [], cols=True)
