@@ -102,15 +102,19 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     int rc;
     if ((rc = cols > 1 ? get_plan_cols(rows, &prow) : get_plan(rows, &prow))) return rc;
     if ((rc = cols > 1 ? get_plan_cols(ext, &pext) : get_plan(ext, &pext))) return rc;
-    // extension: the coset shift by 2717^j and the zero rows j >= rows are
-    // folded into the first extension pass when it is a two-sub-step kernel
-    // (it then reads coef directly); otherwise a separate kernel builds the
-    // input.  When the intt's last pass is a single kernel as well, the
-    // shift joins the intt's 1/rows scale (one product per output either
-    // way) and the extension pass only zero-extends.
-    const bool fold = !pext->passes.empty() && pext->passes[0].v->kind == VK_TWO_STAGE;
+    // extension input (coef shifted by 2717^j, zero rows j >= rows): when the
+    // intt's last pass is a single kernel, the shift joins its 1/rows scale
+    // (one product per output either way) and the extension's first pass
+    // (two-sub-step or three-stage) reads coef directly, treating the upper
+    // rows as zero; otherwise a two-sub-step first extension pass applies
+    // the shift itself; otherwise a separate kernel builds the input.
+    const int first_kind = pext->passes.empty() ? (int)VK_COMPOSITE : pext->passes[0].v->kind;
     const int last_kind = prow->passes.empty() ? (int)VK_COMPOSITE : prow->passes.back().v->kind;
-    const bool fold_scale = fold && (last_kind == VK_TWO_STAGE || last_kind == VK_THREE_STAGE);
+    const bool single = last_kind == VK_TWO_STAGE || last_kind == VK_THREE_STAGE;
+    // zero extension alone: two-sub-step and three-stage first passes; the
+    // shift as well: two-sub-step ones
+    const bool fold_scale = single && (first_kind == VK_TWO_STAGE || first_kind == VK_THREE_STAGE);
+    const bool fold = fold_scale || first_kind == VK_TWO_STAGE;
     const u64 *ct = fold && !fold_scale ? coset_table(rows) : nullptr;
     const u64 *sct = fold_scale ? scaled_coset_table(rows) : nullptr;
     if ((fold && !fold_scale && !ct) || (fold_scale && !sct))

@@ -268,10 +268,18 @@ __device__ __forceinline__ void pass3_body(const PassArgs &a, u32 blk, u64 *lds)
                     u64 pos = (u64)m + (u64)a.Mp * (u64)(r1 * R23 + r2 * R3 + r3);
                     x[r1] = src[s + a.S * (pos ? a.n_rev - pos : 0)];
                 }
+            } else if (a.flags & FLAG_ZEXT) {
+                // zero-extended column: rows j >= coset_rows read as zero
+                const u64 *p = src + (u64)i + a.SMp * (u64)(r2 * R3 + r3);
+#pragma unroll
+                for (int r1 = 0; r1 < R1; r1++) {
+                    const u64 j = (u64)m + (u64)a.Mp * (u64)(r1 * R23 + r2 * R3 + r3);
+                    x[r1] = j < a.coset_rows ? *at_offset(p, uniform_offset(a.SMp, r1 * R23)) : 0;
+                }
             } else {
                 const u64 *p = src + (u64)i + a.SMp * (u64)(r2 * R3 + r3);
 #pragma unroll
-                for (int r1 = 0; r1 < R1; r1++) x[r1] = p[a.SMp * (u64)(r1 * R23)];
+                for (int r1 = 0; r1 < R1; r1++) x[r1] = *at_offset(p, uniform_offset(a.SMp, r1 * R23));
             }
         }
         dft<R1>(x);

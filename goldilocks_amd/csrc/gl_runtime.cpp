@@ -764,7 +764,9 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
     if ((coset || coset_rows) && p == 0) {
-        if (pp.v->kind != VK_TWO_STAGE) return set_error(GL_E_ARG, "coset input needs a two-sub-step first pass");
+        // coset shift: two-sub-step kernels; zero extension alone: also three-stage ones
+        if (pp.v->kind != VK_TWO_STAGE && (coset || pp.v->kind != VK_THREE_STAGE))
+            return set_error(GL_E_ARG, "coset input needs a two-sub-step first pass");
         a.flags |= coset ? FLAG_COSET : FLAG_ZEXT;
         a.coset = coset;
         a.coset_rows = coset_rows;

@@ -15,14 +15,16 @@ def gl():
 
 
 # Shapes cover the three ways the extension's input is built: the coset
-# shift in the intt's last-pass row scale (most shapes), the shift in the
-# first extension pass when the intt ends with the radix-257 Rader kernel
-# (2^10 * 257 rows), and the separate coset kernel when the extension starts
-# with a pass that is not a two-sub-step kernel (65537 rows).
+# shift in the intt's last-pass row scale (most shapes; with the tuned
+# column schedules 2^14 x 16 and 2^17 x 4 also run three-stage passes last
+# in the intt and first in the extension), the shift in the first extension
+# pass when the intt ends with the radix-257 Rader kernel (2^10 * 257 rows),
+# and the separate coset kernel when the extension starts with a pass that
+# is neither (65537 rows).
 @pytest.mark.parametrize("rows,cols,blowup", [(16, 4, 2), (8, 3, 2), (1024, 64, 2), (4096, 7, 4),
                                               (2**14, 16, 2), (240, 5, 2), (3 * 2**10, 2, 2), (1, 4, 2),
                                               (257 * 4, 3, 2), (17 * 16, 5, 2), (65537, 1, 2), (2**18, 1, 2),
-                                              (2**10 * 257, 1, 2)])
+                                              (2**10 * 257, 1, 2), (2**17, 4, 2)])
 def test_commit_host_vs_oracle(gl, oracle, rows, cols, blowup):
     note(f"commit {rows}x{cols} blowup {blowup}")
     m = oracle.fill_random(rows * cols, rows * 31 + cols).reshape(rows, cols)
