@@ -36,6 +36,13 @@ __device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
     return mont_mul(a.tw_lo[e & ((1ull << a.lo_bits) - 1)], a.tw_hi[e >> a.lo_bits]);
 }
 
+// v[j] -> dst[j * dstep] for j < NK.
+template <int NK>
+__device__ __forceinline__ void store_all(u64 *dst, u64 dstep, const u64 *v) {
+#pragma unroll
+    for (int j = 0; j < NK; j++) *at_offset(dst, uniform_offset(dstep, j)) = v[j];
+}
+
 // Store the NK outputs v[j] (output index k = k0 + j*kstep) of instance m,
 // applying the pass's output twiddle w_M^(m*k) and scale.  With the full
 // table each twiddle is one load at a compile-time offset from the
@@ -43,12 +50,6 @@ __device__ __forceinline__ u64 tw_pow(const PassArgs &a, u64 e) {
 // A * B^j with A = w^(m*k0), B = w^(m*kstep): two table lookups per thread,
 // then one product per output to advance (exact field arithmetic, so the
 // result equals the per-output lookup).
-template <int NK>
-__device__ __forceinline__ void store_all(u64 *dst, u64 dstep, const u64 *v) {
-#pragma unroll
-    for (int j = 0; j < NK; j++) *at_offset(dst, uniform_offset(dstep, j)) = v[j];
-}
-
 template <int NK, int KSTEP>
 __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 dstep, u32 s, u32 m, u32 k0, u32 R,
                                               u64 *v) {
