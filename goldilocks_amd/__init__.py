@@ -274,8 +274,9 @@ def timing_collect():
 
 def set_twiddle_full_max(words):
     """Threshold (sub-problem length) up to which a pass's output twiddles
-    come from one full table; larger passes use the two-level power table.
-    Results are identical; returns the previous value."""
+    come from one full table (up to 4x that for passes whose instance stride
+    is >= 32); larger passes use the two-level power table.  Results are
+    identical; returns the previous value."""
     return lib().gl_set_twiddle_full_max(int(words))
 
 

@@ -97,12 +97,214 @@ template <int R2>
 constexpr int lds_rs() { return R2 | 1; }
 template <int R1, int R2>
 constexpr int lds_ts() { return (R1 * lds_rs<R2>()) | 1; }
+
+// Exchange through LDS in 32-bit halves (low words, then high words): half
+// the LDS of the 64-bit tile, so twice the workgroups fit per CU, for two
+// more barriers and the high words held in registers meanwhile.  Used for
+// radices >= GL_HALF_EXCHANGE_MIN_R (512: the 1024-point 32x32 pass on a
+// 2^28-word transform ran 1783 -> 1423 us, the 256-point passes gained
+// nothing) where each thread has one task per sub-step.
+#ifndef GL_HALF_EXCHANGE_MIN_R
+#define GL_HALF_EXCHANGE_MIN_R 512
+#endif
+template <int R1, int R2, int NT, int T>
+constexpr bool half_exchange() {
+    return GL_HALF_EXCHANGE_MIN_R > 0 && R2 > 1 && R1 * R2 >= GL_HALF_EXCHANGE_MIN_R && T * R2 <= NT &&
+           T * R1 <= NT && R1 + R2 <= 64;
+}
+
+// Worst bank multiplicity (32 banks, per 32-lane half wave) of the 32-bit
+// tile's three access patterns: sub-step a writes (lanes over t, then rb),
+// sub-step b reads with S == 1 (lanes over ka, then t) and with S > 1
+// (lanes over t, then ka).
+template <int R1, int T>
+constexpr int lds32_conflicts(int RS, int TS) {
+    int worst = 0;
+    for (int pat = 0; pat < 3; pat++) {
+        for (int h = 0; h < 2; h++) {
+            int cnt[32] = {};
+            for (int l = 0; l < 32; l++) {
+                const int L = h * 32 + l;
+                const int addr = pat == 0 ? (L % T) * TS + L / T
+                               : pat == 1 ? (L / R1) * TS + (L % R1) * RS
+                                          : (L % T) * TS + (L / T) * RS;
+                const int c = ++cnt[addr % 32];
+                worst = c > worst ? c : worst;
+            }
+        }
+    }
+    return worst;
+}
+// Instance stride of the 32-bit tile: the least padding with the fewest
+// conflicts (rows keep the odd stride R2 | 1).
 template <int R1, int R2, int T>
-constexpr int lds_words() { return R2 > 1 ? T * lds_ts<R1, R2>() : 1; }
+constexpr int lds32_ts() {
+    constexpr int RS = lds_rs<R2>();
+    int best = R1 * RS, fewest = 1 << 30;
+    for (int ts = R1 * RS; ts < R1 * RS + 64; ts++) {
+        const int c = lds32_conflicts<R1, T>(RS, ts);
+        if (c < fewest) {
+            fewest = c;
+            best = ts;
+        }
+    }
+    return best;
+}
+
+// Waves per SIMD the half-exchange kernels are compiled for (register cap).
+#ifndef GL_HALF_MIN_WAVES
+#define GL_HALF_MIN_WAVES 1
+#endif
+template <int R1, int R2, int NT, int T>
+constexpr int pass_min_waves() { return half_exchange<R1, R2, NT, T>() ? GL_HALF_MIN_WAVES : 1; }
+
+template <int R1, int R2, int NT, int T>
+constexpr int lds_words() {
+    if constexpr (half_exchange<R1, R2, NT, T>()) return (T * lds32_ts<R1, R2, T>() + 1) / 2;
+    return R2 > 1 ? T * lds_ts<R1, R2>() : 1;
+}
+
+// Sub-step a of the task (instance I, row rb): the R1 inputs of row rb,
+// DFT_R1 and, with R2 > 1, the inner twiddles w_R^(ka*rb).  b, i, s, m
+// locate the instance; lanes past the last instance load the last
+// instance's words (their results are never stored), so the loads need no
+// branch and no zero fill.  pass_body keeps the same steps inline: calling
+// this helper there raised its register use (82 -> 110 VGPRs for 16x16).
+template <int R1, int R2>
+__device__ __forceinline__ void substep_a(const PassArgs &a, u32 I, int rb, u64 *x, u32 &b, u32 &i, u32 &s,
+                                          u32 &m) {
+    const bool valid = I < a.total_inst;
+    // inner twiddles: row rb of the table, issued before the data loads so
+    // one wait covers both
+    u64 w[R1];
+    if constexpr (R2 > 1) {
+        const u64 *twr = a.tw_inner + rb * R1;
+#pragma unroll
+        for (int ka = 1; ka < R1; ka++) w[ka] = twr[ka];
+    }
+    locate(a, valid ? I : a.total_inst - 1, b, i, s, m);
+    const u64 *src = a.in + (u64)b * a.bstride;
+    if (a.flags & FLAG_REVERSE) {
+#pragma unroll
+        for (int ra = 0; ra < R1; ra++) {
+            u64 pos = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+            u64 rpos = pos ? a.n_rev - pos : 0;
+            x[ra] = src[s + a.S * rpos];
+        }
+    } else if (a.flags & FLAG_COSET) {
+        // input row j = m + Mp*r of a zero-extended, coset-shifted
+        // column: coef[j] * g^j below coset_rows, zero above
+        const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+        const u64 step = a.SMp * (u64)R2;
+#pragma unroll
+        for (int ra = 0; ra < R1; ra++) {
+            const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+            x[ra] = j < a.coset_rows ? mont_mul(*at_offset(p, uniform_offset(step, ra)), a.coset[j]) : 0;
+        }
+    } else if (a.flags & FLAG_ZEXT) {
+        // zero-extended column: rows j >= coset_rows read as zero
+        const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+        const u64 step = a.SMp * (u64)R2;
+#pragma unroll
+        for (int ra = 0; ra < R1; ra++) {
+            const u64 j = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
+            x[ra] = j < a.coset_rows ? *at_offset(p, uniform_offset(step, ra)) : 0;
+        }
+    } else {
+        const u64 *p = src + (u64)i + a.SMp * (u64)rb;
+        const u64 step = a.SMp * (u64)R2;
+#pragma unroll
+        for (int ra = 0; ra < R1; ra++) x[ra] = *at_offset(p, uniform_offset(step, ra));
+    }
+    dft<R1>(x);
+    if constexpr (R2 > 1) {
+#pragma unroll
+        for (int ka = 1; ka < R1; ka++) x[ka] = mont_mul(x[ka], w[ka]);
+    }
+}
+
+// Sub-step b of the task (instance at b, s, m; column ka): DFT_R2 over the
+// column and the stores of outputs k = ka + R1*kb.
+template <int R1, int R2>
+__device__ __forceinline__ void substep_b(const PassArgs &a, bool valid, u32 b, u32 s, u32 m, int ka, u64 *z) {
+    constexpr int R = R1 * R2;
+    dft<R2>(z);
+    if (valid) {
+        u64 *dst = a.out + (u64)b * a.bstride + s + a.S * ((u64)ka + (u64)R * (u64)m);
+        store_outputs<R2, R1>(a, dst, a.S * (u64)R1, s, m, (u32)ka, (u32)R, z);
+    }
+}
+
+// Sub-step b's task for thread slot tb.  S == 1: an instance's outputs are
+// contiguous, so lanes run over ka first; otherwise over consecutive
+// instances.
+template <int R1, int T>
+__device__ __forceinline__ void task_b(const PassArgs &a, int tb, int &t, int &ka) {
+    if (a.S == 1) {
+        ka = tb % R1;
+        t = tb / R1;
+    } else {
+        t = tb % T;
+        ka = tb / T;
+    }
+}
+
+// Two-sub-step pass exchanging 32-bit halves, one task per thread and
+// sub-step: sub-step a writes the low words and keeps the high words in
+// registers; sub-step b reads the low words, then (after the high words
+// replace them) the high words.
+template <int R1, int R2, int NT, int T>
+__device__ __forceinline__ void pass_body_half(const PassArgs &a, u32 blk, u32 *lds) {
+    constexpr int RS = lds_rs<R2>(), TS = lds32_ts<R1, R2, T>();
+    const int tid = threadIdx.x;
+    const u32 I0 = blk * (u32)T;
+    const bool act_a = tid < T * R2, act_b = tid < T * R1;
+    u32 *wrow = lds + (tid % T) * TS + tid / T;
+    u32 xh[R1];
+    if (act_a) {
+        u64 x[R1];
+        u32 b, i, s, m;
+        substep_a<R1, R2>(a, I0 + (u32)(tid % T), tid / T, x, b, i, s, m);
+#pragma unroll
+        for (int ka = 0; ka < R1; ka++) {
+            wrow[ka * RS] = lo32(x[ka]);
+            xh[ka] = hi32(x[ka]);
+        }
+    }
+    __syncthreads();
+    int t, ka;
+    task_b<R1, T>(a, tid, t, ka);
+    const u32 I = I0 + (u32)t;
+    const bool valid = act_b && I < a.total_inst;
+    u32 b = 0, i = 0, s = 0, m = 0;
+    if (valid) locate(a, I, b, i, s, m);
+    const u32 *rrow = lds + t * TS + ka * RS;
+    u32 zl[R2];
+    if (act_b) {
+#pragma unroll
+        for (int rb = 0; rb < R2; rb++) zl[rb] = rrow[rb];
+    }
+    __syncthreads();
+    if (act_a) {
+#pragma unroll
+        for (int ka2 = 0; ka2 < R1; ka2++) wrow[ka2 * RS] = xh[ka2];
+    }
+    __syncthreads();
+    if (act_b) {
+        u64 z[R2];
+#pragma unroll
+        for (int rb = 0; rb < R2; rb++) z[rb] = mk64(zl[rb], rrow[rb]);
+        substep_b<R1, R2>(a, valid, b, s, m, ka, z);
+    }
+}
 
 // One workgroup's share of a pass: instances [blk*T, blk*T + T).
 template <int R1, int R2, int NT, int T>
 __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) {
+    if constexpr (half_exchange<R1, R2, NT, T>()) {
+        pass_body_half<R1, R2, NT, T>(a, blk, (u32 *)lds);
+        return;
+    }
     constexpr int R = R1 * R2;
     constexpr int RS = lds_rs<R2>(), TS = lds_ts<R1, R2>();
     const int tid = threadIdx.x;
@@ -373,8 +575,8 @@ static void launch_pass3_grouped(const PassArgs *descs, const u32 *starts, u32 n
 }
 
 template <int R1, int R2, int NT, int T>
-__global__ __launch_bounds__(NT) void ntt_pass_kernel(PassArgs a) {
-    __shared__ u64 lds[lds_words<R1, R2, T>()];
+__global__ __launch_bounds__(NT, (pass_min_waves<R1, R2, NT, T>())) void ntt_pass_kernel(PassArgs a) {
+    __shared__ u64 lds[lds_words<R1, R2, NT, T>()];
     pass_body<R1, R2, NT, T>(a, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
@@ -382,9 +584,9 @@ __global__ __launch_bounds__(NT) void ntt_pass_kernel(PassArgs a) {
 // to run the same pass variant of many transforms (of different lengths)
 // in one launch.
 template <int R1, int R2, int NT, int T>
-__global__ __launch_bounds__(NT) void ntt_pass_grouped_kernel(const PassArgs *__restrict__ descs,
+__global__ __launch_bounds__(NT, (pass_min_waves<R1, R2, NT, T>())) void ntt_pass_grouped_kernel(const PassArgs *__restrict__ descs,
                                                               const u32 *__restrict__ starts, u32 nd) {
-    __shared__ u64 lds[lds_words<R1, R2, T>()];
+    __shared__ u64 lds[lds_words<R1, R2, NT, T>()];
     const u32 b = blockIdx.x;
     u32 lo = 0, hi = nd;  // find the last j with starts[j] <= b
     while (hi - lo > 1) {
@@ -416,7 +618,7 @@ constexpr int pick_T() {
 }
 
 #define GL_VARIANT_T(R1, R2, NT, T)                                                          \
-    {R1, R2, T, NT, (size_t)lds_words<R1, R2, T>() * 8, &launch_pass<R1, R2, NT, T>,         \
+    {R1, R2, T, NT, (size_t)lds_words<R1, R2, NT, T>() * 8, &launch_pass<R1, R2, NT, T>,         \
      &launch_grouped<R1, R2, NT, T>, VK_TWO_STAGE, 1}
 #define GL_VARIANT(R1, R2) GL_VARIANT_T(R1, R2, 256, (pick_T<R1, R2, 256>()))
 // Three-stage variants: the R2 field holds R2*R3.

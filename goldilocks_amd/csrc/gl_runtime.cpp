@@ -248,7 +248,8 @@ const u64 *twiddle_matrix(u64 M, u64 R) {
 // A full twiddle matrix costs M words and one read per output; it replaces
 // the two-level product (one multiply per output).  It pays off while the
 // matrix stays cache resident (M up to 2^20 words = 8 MiB by default, set by
-// GOLDILOCKS_TWIDDLE_FULL_MAX); larger passes use the two-level tables.
+// GOLDILOCKS_TWIDDLE_FULL_MAX), or while adjacent instances share its words
+// (see PassCursor::next); other passes use the two-level tables.
 static std::atomic<u64> &full_table_max_ref() {
     static std::atomic<u64> v{[] {
         const char *e = getenv("GOLDILOCKS_TWIDDLE_FULL_MAX");
@@ -749,7 +750,12 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
                              (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1);
     if (needs_inner && !a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     if (Mp > 1) {
-        const u64 *full = M <= full_table_max() ? twiddle_matrix(M, R) : nullptr;
+        // Past the cache-resident size a full matrix still pays off when S >= 32:
+        // the S adjacent instances s + S*m share each of its words, so every
+        // workgroup fetches one run of R words for T*R data words.
+        const u64 fmax = full_table_max();
+        const bool use_full = M <= fmax || (S >= 32 && M <= 4 * fmax);
+        const u64 *full = use_full ? twiddle_matrix(M, R) : nullptr;
         if (full) {
             a.tw_lo = full;
             a.flags |= FLAG_TWIDDLE_FULL;

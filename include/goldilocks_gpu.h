@@ -133,10 +133,11 @@ int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
 int gl_plan_variants_cols(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses);
 
 /* Output twiddles of a pass on sub-problems of length M use one M-word table
- * (one load and one product per output) when M <= words, else a two-level
- * power table (w^(m*k0) * (w^(m*kstep))^j).  Results are identical either
- * way; returns the previous threshold (default 2^20, or
- * GOLDILOCKS_TWIDDLE_FULL_MAX). */
+ * (one load and one product per output) when M <= words, or M <= 4 * words
+ * for passes whose instance stride S is >= 32 (S adjacent instances share
+ * each table word), else a two-level power table (w^(m*k0) * (w^(m*kstep))^j).
+ * Results are identical either way; returns the previous threshold (default
+ * 2^20, or GOLDILOCKS_TWIDDLE_FULL_MAX; 0 disables the full tables). */
 uint64_t gl_set_twiddle_full_max(uint64_t words);
 
 /* Pass timing (measurement aid, no reference counterpart).  While enabled,

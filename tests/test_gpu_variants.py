@@ -65,6 +65,31 @@ def test_every_variant_every_position(gl, oracle, full_max):
         gl.set_twiddle_full_max(prev)
 
 
+@pytest.mark.parametrize("full_max", [0, 1 << 10, 1 << 20], ids=["two-level", "shared-full", "full"])
+def test_column_twiddle_tables(gl, oracle, full_max):
+    """Column transforms of 2^12 rows: with threshold 2^10 the 64-column passes
+    (instance stride >= 32) take the full table though M > threshold, the
+    16-column ones the two-level table; every column stays bit-exact."""
+    import torch
+    n = 4096
+    prev = gl.set_twiddle_full_max(full_max)
+    try:
+        for cols in (64, 16):
+            note(f"column twiddle tables n={n} cols={cols} full_max={full_max}")
+            m = oracle.fill_random(n * cols, 31 + cols).reshape(n, cols)
+            t = to_torch(m.reshape(-1)).reshape(n, cols)
+            gl.ntt_cols(t)
+            torch.cuda.synchronize()
+            got = from_torch(t.reshape(-1)).reshape(n, cols)
+            for c in range(cols):
+                assert np.array_equal(got[:, c], oracle.ntt(np.ascontiguousarray(m[:, c]))), (cols, c)
+            gl.ntt_cols(t, inverse=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(from_torch(t.reshape(-1)).reshape(n, cols), m)
+    finally:
+        gl.set_twiddle_full_max(prev)
+
+
 def test_every_variant_grouped_batch(gl, oracle):
     """The same schedules as one grouped launch sequence over device arrays."""
     vs = gl.variants()
