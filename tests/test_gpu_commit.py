@@ -52,3 +52,27 @@ def test_commit_device_medium(gl, oracle):
 def test_commit_errors(gl):
     with pytest.raises(ValueError):
         gl.commit(np.zeros((7, 2), dtype=np.uint64), 2)
+
+
+@pytest.mark.parametrize("rows,cols", [(1024, 5), (4096, 64)])
+def test_commit_half_exchange_passes(gl, oracle, rows, cols):
+    """Column schedules whose intt ends (reversed reads and row scale when it
+    is one pass) and whose extension starts (zero extension) with the
+    1024-point pass that exchanges 32-bit halves through LDS."""
+    vs = {v["name"]: v["id"] for v in gl.variants()}
+    k32 = vs["ntt_pass_kernel<32, 32, 256, 8>"]
+    k2, k4 = vs["ntt_pass_kernel<2, 1, 256, 256>"], vs["ntt_pass_kernel<4, 1, 256, 256>"]
+    plans = {1024: [k32], 2048: [k32, k2], 4096: [k4, k32], 8192: [k32, k4, k2]}
+    try:
+        for n in (rows, 2 * rows):
+            gl.plan_set(n, plans[n], cols=True)
+            assert gl.plan_ids(n, cols=True) == plans[n]
+        note(f"commit {rows}x{cols} through half-exchange column passes")
+        m = oracle.fill_random(rows * cols, rows + 7 * cols).reshape(rows, cols)
+        lde, layers = gl.commit(m, 2)
+        lde_o, layers_o = oracle.commit(m, 2)
+        assert np.array_equal(lde, lde_o)
+        assert layers == layers_o
+    finally:
+        for n in (rows, 2 * rows):
+            gl.plan_set(n, [], cols=True)
