@@ -221,8 +221,8 @@ def bench_ntt(args, torch, gl):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the library "
+              "runs the transforms and the commit on a single MI355X", file=sys.stderr)
         sys.exit(2)
     import torch
     import goldilocks_amd as gl
