@@ -4,13 +4,16 @@
 // a_q = x[86^q mod 257], A = DFT_256(a), X0 = x0 + A[0], C = A * TW
 // (TW = DFT_256(w_257^(3^q) / 256), precomputed), C[0] += x0, D =
 // DFT_256(C reversed on [1..]), X[3^q mod 257] = D[q].  Generators
-// (gi, gk) = (3, 86) as rader.rs:21-29.  Both DFT_256 run as 16 x 16 with
-// register codelets and one LDS tile; T = 16 instances per workgroup, one
-// DFT_16 task per thread per sub-step.  The pass then applies the usual
-// output twiddle w_M^(m*k) and Stockham store (see PassArgs).
+// (gi, gk) = (3, 86) as rader.rs:21-29.  Both 256-point transforms run as
+// 16 x 16 with register codelets; the second is taken as the unscaled
+// inverse DFT of C (the same numbers), which starts on the registers the
+// first one ends in, so the instance crosses the LDS tile twice, not three
+// times.  T = 16 instances per workgroup, one DFT_16 task per thread per
+// sub-step.  The pass then applies the usual output twiddle w_M^(m*k) and
+// Stockham store (see PassArgs).
 //
-// a.tw_inner holds w_256^(k1*j) at j*16 + k1 and the spectrum TW at
-// 256 + k (Montgomery form).  The Rader permutations 86^q and 3^q mod 257
+// a.tw_inner holds w_256^(k1*j) at j*16 + k1, the spectrum TW at 256 + k
+// and w_256^(-j*q1) at 512 + j*16 + q1 (Montgomery form).  The Rader permutations 86^q and 3^q mod 257
 // are computed in registers: 86^(16*q1 + j) = 86^(16*q1) * 86^j, the first
 // factor a compile-time constant, reduced with 2^8 = -1 (mod 257).
 #include "gl_dft.hpp"
@@ -110,28 +113,26 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *l
         }
         x[k2] = c;
     }
+    // sub-step c: D = DFT_256(C reversed on [1..]) is the unscaled inverse
+    // DFT of C: D[q1 + 16 q2] = sum_j w_16^(-j q2) w_256^(-j q1) F_j[q1],
+    // F_j[q1] = sum_k2 C[j + 16 k2] w_16^(-k2 q1).  Thread j holds
+    // C[j + 16 k2], so F_j is its own DFT_16 read at -q1: no exchange.
+    const u64 *winv = a.tw_inner + 512 + j * 16;
+#pragma unroll
+    for (int q1 = 1; q1 < 16; q1++) w[q1] = winv[q1];
+    dft<16>(x);
+    u64 g[16];
+    g[0] = x[0];
+#pragma unroll
+    for (int q1 = 1; q1 < 16; q1++) g[q1] = mont_mul(x[16 - q1], w[q1]);
     __syncthreads();  // every sub-step b read is done before the tile is reused
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) {
-        const u32 p = (256u - (u32)(j + 16 * k2)) & 255u;
-        row[(p & 15u) * RD_RS + (p >> 4)] = x[k2];
-    }
+    for (int q1 = 0; q1 < 16; q1++) row[q1 * RD_RS + j] = g[q1];
     __syncthreads();
-    // sub-step c: DFT_16 over q1 of D_in[q1*16 + j] (row j of the tile)
+    // sub-step d: thread j (as q1) transforms column j of the tile over the
+    // rows, read at -q2: D[j + 16 q2] = x[-q2] -> X[3^(j + 16 q2) mod 257]
 #pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) w[k1] = w256[k1];
-#pragma unroll
-    for (int q1 = 0; q1 < 16; q1++) x[q1] = row[j * RD_RS + q1];
-    dft<16>(x);
-#pragma unroll
-    for (int k1 = 1; k1 < 16; k1++) x[k1] = mont_mul(x[k1], w[k1]);
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < 16; k1++) row[k1 * RD_RS + j] = x[k1];
-    __syncthreads();
-    // sub-step d: D[j + 16 k2] -> X[3^(j + 16 k2) mod 257]
-#pragma unroll
-    for (int q2 = 0; q2 < 16; q2++) x[q2] = row[j * RD_RS + q2];
+    for (int r = 0; r < 16; r++) x[r] = row[j * RD_RS + r];
     dft<16>(x);
     if (!valid) return;
     u64 *dst = a.out + (u64)b * a.bstride + s + a.S * 257ull * (u64)m;
@@ -147,7 +148,7 @@ __device__ __forceinline__ void rader257_body(const PassArgs &a, u32 blk, u64 *l
     };
     const u32 hi = pow257_small<3>((u32)j);
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) store(mod257(kGi16.v[k2] * hi), x[k2]);
+    for (int k2 = 0; k2 < 16; k2++) store(mod257(kGi16.v[k2] * hi), x[(16 - k2) & 15]);
     if (j == 0) store(0, X0);
 }
 

@@ -354,7 +354,7 @@ const u64 *inner_matrix(u64 R1, u64 R2) {
 const u64 *rader257_table() {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     if (g_rader257) return g_rader257;
-    std::vector<u64> pw(256), t(512);
+    std::vector<u64> pw(256), t(768);
     u64 w256 = root(256), w257 = root(257), s256 = inv(256);
     u64 v = 1;
     for (int e = 0; e < 256; e++) {
@@ -374,7 +374,9 @@ const u64 *rader257_table() {
         for (int q = 0; q < 256; q++) acc = add(acc, mul(c[q], pw[(q * k) & 255]));
         t[256 + k] = acc;
     }
-    for (int e = 0; e < 512; e++) t[e] = to_mont(t[e]);
+    for (int j = 0; j < 16; j++)
+        for (int q1 = 0; q1 < 16; q1++) t[512 + j * 16 + q1] = pw[(256 - ((q1 * j) & 255)) & 255];
+    for (int e = 0; e < 768; e++) t[e] = to_mont(t[e]);
     g_rader257 = upload(t);
     return g_rader257;
 }
