@@ -85,6 +85,30 @@ GL_HD void dft_pow2(u64 *x) {
     for (int k = 0; k < N; k++) x[k] = t[k];
 }
 
+// dft_pow2 for inputs whose upper half x[N/2 .. N) is zero (and is not
+// read): the first layer's butterflies (a + 0, (a - 0) w_N^J) reduce to
+// x[J + N/2] = x[J] w_N^J; the results are the same canonical words.
+template <int N, int J>
+GL_HD void dif_layer_upper_zero(u64 *x) {
+    if constexpr (J < N / 2) {
+        x[J + N / 2] = shl<(192 / N) * J>(x[J]);
+        dif_layer_upper_zero<N, J + 1>(x);
+    }
+}
+
+template <int N>
+GL_HD void dft_pow2_upper_zero(u64 *x) {
+    static_assert(N <= 64 && 192 % N == 0, "shift-only power-of-two codelet");
+    dif_layer_upper_zero<N, 0>(x);
+    dif<N / 2, 0>(x);
+    dif<N / 2, N / 2>(x);
+    u64 t[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) t[k] = x[bitrev(k, log2i(N))];
+#pragma unroll
+    for (int k = 0; k < N; k++) x[k] = t[k];
+}
+
 // ---- 3: w_3 = 2^64 ----
 GL_HD void dft3(u64 *x) {
     u64 a = x[0], b = x[1], c = x[2];

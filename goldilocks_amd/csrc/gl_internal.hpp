@@ -79,6 +79,7 @@ enum : u32 {
     FLAG_COSET = 16u,  // first pass of the commit's extension: coset shift + zero rows
     FLAG_ZEXT = 32u,   // first pass: rows >= coset_rows read as zero (no shift)
     FLAG_ROW_SCALE = 64u,  // last pass: output row r times row_scale[r] (replaces FLAG_SCALE)
+    FLAG_ZEXT_HALF = 128u,  // with FLAG_ZEXT: coset_rows = M/2, so inputs r >= R/2 are the zero ones
 };
 // All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
 // are applied with mont_mul.

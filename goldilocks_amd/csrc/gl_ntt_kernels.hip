@@ -319,6 +319,25 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
         const int t = ta % T, rb = ta / T;
         const u32 I = I0 + (u32)t;
         const bool valid = I < a.total_inst;
+        if constexpr (R2 == 1 && R1 >= 32 && is_pow2(R1)) {
+            if (a.flags & FLAG_ZEXT_HALF) {
+                // zero extension to twice the rows: inputs r >= R1/2 are the
+                // zero rows, so only the lower half is read and the first
+                // codelet layer needs no additions
+                u64 x[R1];
+                u32 b, i, s, m;
+                locate(a, valid ? I : a.total_inst - 1, b, i, s, m);
+                const u64 *p = a.in + (u64)b * a.bstride + (u64)i;
+#pragma unroll
+                for (int ra = 0; ra < R1 / 2; ra++) x[ra] = *at_offset(p, uniform_offset(a.SMp, ra));
+                dft_pow2_upper_zero<R1>(x);
+                if (valid) {
+                    u64 *dst = a.out + (u64)b * a.bstride + s + a.S * (u64)R * (u64)m;
+                    store_outputs<R1, 1>(a, dst, a.S, s, m, 0, R, x);
+                }
+                continue;
+            }
+        }
         // inner twiddles w_R^(ka*rb): row rb of the table, issued before
         // the data loads so one wait covers both
         u64 w[R1];

@@ -776,6 +776,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         if (pp.v->kind != VK_TWO_STAGE && (coset || pp.v->kind != VK_THREE_STAGE))
             return set_error(GL_E_ARG, "coset input needs a two-sub-step first pass");
         a.flags |= coset ? FLAG_COSET : FLAG_ZEXT;
+        if (!coset && 2 * coset_rows == M) a.flags |= FLAG_ZEXT_HALF;
         a.coset = coset;
         a.coset_rows = coset_rows;
     }

@@ -76,3 +76,27 @@ def test_commit_half_exchange_passes(gl, oracle, rows, cols):
     finally:
         for n in (rows, 2 * rows):
             gl.plan_set(n, [], cols=True)
+
+
+@pytest.mark.parametrize("rows,cols,first", [(1024, 5, "32, 1"), (2048, 64, "64, 1"), (1024, 3, "16, 1")])
+def test_commit_single_stage_extension_pass(gl, oracle, rows, cols, first):
+    """The extension starts with a single-sub-step pass; with blowup 2 its
+    inputs r >= R/2 are exactly the zero rows, which the 32- and 64-point
+    passes skip (lower half read, first codelet layer without additions)."""
+    vs = {v["name"]: v["id"] for v in gl.variants()}
+    k1 = vs[f"ntt_pass_kernel<{first}, 256, 256>"]
+    r1 = int(first.split(",")[0])
+    rest = {2048 // 32: [vs["ntt_pass_kernel<8, 8, 256, 32>"]],
+            4096 // 64: [vs["ntt_pass_kernel<8, 8, 256, 32>"]],
+            2048 // 16: [vs["ntt_pass_kernel<16, 8, 256, 32>"]]}[2 * rows // r1]
+    n = 2 * rows
+    try:
+        gl.plan_set(n, [k1] + rest, cols=True)
+        note(f"commit {rows}x{cols}, extension first pass <{first}>")
+        m = oracle.fill_random(rows * cols, 3 * rows + cols).reshape(rows, cols)
+        lde, layers = gl.commit(m, 2)
+        lde_o, layers_o = oracle.commit(m, 2)
+        assert np.array_equal(lde, lde_o)
+        assert layers == layers_o
+    finally:
+        gl.plan_set(n, [], cols=True)
