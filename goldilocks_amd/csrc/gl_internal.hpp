@@ -33,7 +33,7 @@ struct PassArgs {
     const u64 *tw_inner;  // w_R^e, e < R
     const u64 *tw_lo;
     const u64 *tw_hi;
-    u64 scale;         // multiplier applied to every output when FLAG_SCALE
+    u64 scale;         // multiplier applied to every output when FLAG_SCALE or FLAG_SCALE_TW
     const u64 *coset;  // FLAG_COSET: input row j < coset_rows is multiplied by coset[j]
     u64 coset_rows;    //   (Montgomery g^j); rows >= coset_rows read as zero (also FLAG_ZEXT)
     const u64 *row_scale;  // FLAG_ROW_SCALE: output row r is multiplied by row_scale[r]
@@ -80,6 +80,7 @@ enum : u32 {
     FLAG_ZEXT = 32u,   // first pass: rows >= coset_rows read as zero (no shift)
     FLAG_ROW_SCALE = 64u,  // last pass: output row r times row_scale[r] (replaces FLAG_SCALE)
     FLAG_ZEXT_HALF = 128u,  // with FLAG_ZEXT: coset_rows = M/2, so inputs r >= R/2 are the zero ones
+    FLAG_SCALE_TW = 256u,   // with FLAG_TWIDDLE: the recurrence starts at scale * w^(m*k0)
 };
 // All twiddle tables and `scale` hold Montgomery forms (w * 2^64 mod p) and
 // are applied with mont_mul.

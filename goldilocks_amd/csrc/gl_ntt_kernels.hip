@@ -77,6 +77,7 @@ __device__ __forceinline__ void store_outputs(const PassArgs &a, u64 *dst, u64 d
         store_all<NK>(dst, dstep, v);
     } else if (a.flags & FLAG_TWIDDLE) {
         u64 w = tw_pow(a, (u64)m * k0);
+        if (a.flags & FLAG_SCALE_TW) w = mont_mul(w, a.scale);
         const u64 B = tw_pow(a, (u64)m * KSTEP);
 #pragma unroll
         for (int j = 0; j < NK; j++) {

@@ -768,6 +768,15 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
             a.tw_hi = t.hi;
             a.lo_bits = t.lo_bits;
             a.flags |= FLAG_TWIDDLE;
+            // inverse: 1/n multiplies the first output twiddle of each thread's
+            // recurrence (one product per thread) instead of every output of
+            // the last pass
+            if (inverse && p == 0 && scale_out && !row_scale && np > 1 &&
+                (pp.v->kind == VK_TWO_STAGE || pp.v->kind == VK_THREE_STAGE)) {
+                a.flags |= FLAG_SCALE_TW;
+                a.scale = to_mont(inv(n % P));
+                scale_folded = true;
+            }
         }
     }
     if (inverse && p == 0) a.flags |= FLAG_REVERSE;
@@ -785,7 +794,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
             return set_error(GL_E_ARG, "row scale needs a single-kernel last pass");
         a.flags |= FLAG_ROW_SCALE;
         a.row_scale = row_scale;
-    } else if (inverse && last && scale_out) {
+    } else if (inverse && last && scale_out && !scale_folded) {
         a.flags |= FLAG_SCALE;
         a.scale = to_mont(inv(n % P));
     }

@@ -105,7 +105,8 @@ struct PassCursor {
     size_t p = 0;
     u64 S = 0, M = 0;
     const u64 *src = nullptr;
-    void begin() { p = 0; S = S0; M = plan->n; src = input ? input : data; }
+    bool scale_folded = false;  // 1/n went into the first pass's twiddle recurrence
+    void begin() { p = 0; S = S0; M = plan->n; src = input ? input : data; scale_folded = false; }
     bool done() const { return p >= plan->passes.size(); }
     // Fills args/grid for the current pass and advances; returns GL_OK or an error.
     int next(PassArgs &a, unsigned &grid);
