@@ -100,7 +100,7 @@ template <int R1, int R2>
 constexpr int lds_ts() { return (R1 * lds_rs<R2>()) | 1; }
 
 // Exchange through LDS in 32-bit halves (low words, then high words): half
-// the LDS of the 64-bit tile, so twice the workgroups fit per CU, for two
+// the LDS of the 64-bit tile, so up to twice the workgroups fit per CU, for two
 // more barriers and the high words held in registers meanwhile.  Used for
 // radices >= GL_HALF_EXCHANGE_MIN_R (512: the 1024-point 32x32 pass on a
 // 2^28-word transform ran 1783 -> 1423 us, the 256-point passes gained
@@ -152,7 +152,9 @@ constexpr int lds32_ts() {
     return best;
 }
 
-// Waves per SIMD the half-exchange kernels are compiled for (register cap).
+// Waves per SIMD the half-exchange kernels are compiled for (register cap;
+// 1 = none: the 1024-point pass takes 147 VGPRs, three waves, and a cap of
+// four waves made it spill and run 25-50% slower).
 #ifndef GL_HALF_MIN_WAVES
 #define GL_HALF_MIN_WAVES 1
 #endif
