@@ -188,12 +188,13 @@ __device__ __forceinline__ void substep_a(const PassArgs &a, u32 I, int rb, u64 
     locate(a, valid ? I : a.total_inst - 1, b, i, s, m);
     const u64 *src = a.in + (u64)b * a.bstride;
     if (a.flags & FLAG_REVERSE) {
+        // as in pass_body: a descending run, pos = 0 read at 0
+        const u64 step = a.SMp * (u64)R2;
+        const u64 *p = src + s + a.S * (a.n_rev - (u64)m - (u64)a.Mp * (u64)rb);
+        const u64 *q0 = (m == 0 && rb == 0) ? src + s : p;
+        x[0] = *q0;
 #pragma unroll
-        for (int ra = 0; ra < R1; ra++) {
-            u64 pos = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
-            u64 rpos = pos ? a.n_rev - pos : 0;
-            x[ra] = src[s + a.S * rpos];
-        }
+        for (int ra = 1; ra < R1; ra++) x[ra] = *at_offset(p, 0 - uniform_offset(step, ra));
     } else if (a.flags & FLAG_COSET) {
         // input row j = m + Mp*r of a zero-extended, coset-shifted
         // column: coef[j] * g^j below coset_rows, zero above
@@ -358,12 +359,15 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, u32 blk, u64 *lds) 
             locate(a, valid ? I : a.total_inst - 1, b, i, s, m);
             const u64 *src = a.in + (u64)b * a.bstride;
             if (a.flags & FLAG_REVERSE) {
+                // position pos = m + Mp*(ra*R2 + rb) is read at n - pos (0 at
+                // pos = 0): a descending run with the forward path's uniform
+                // step, except pos = 0 (ra = rb = m = 0)
+                const u64 step = a.SMp * (u64)R2;
+                const u64 *p = src + s + a.S * (a.n_rev - (u64)m - (u64)a.Mp * (u64)rb);
+                const u64 *q0 = (m == 0 && rb == 0) ? src + s : p;
+                x[0] = *q0;
 #pragma unroll
-                for (int ra = 0; ra < R1; ra++) {
-                    u64 pos = (u64)m + (u64)a.Mp * (u64)(ra * R2 + rb);
-                    u64 rpos = pos ? a.n_rev - pos : 0;
-                    x[ra] = src[s + a.S * rpos];
-                }
+                for (int ra = 1; ra < R1; ra++) x[ra] = *at_offset(p, 0 - uniform_offset(step, ra));
             } else if (a.flags & FLAG_COSET) {
                 // input row j = m + Mp*r of a zero-extended, coset-shifted
                 // column: coef[j] * g^j below coset_rows, zero above
