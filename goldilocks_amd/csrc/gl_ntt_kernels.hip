@@ -671,6 +671,9 @@ static const PassVariant kVariants[] = {
     GL_VARIANT_T(64, 64, 128, 2), GL_VARIANT_T(64, 64, 64, 1), GL_VARIANT_T(32, 32, 128, 4),
     GL_VARIANT_T(16, 16, 512, 32), GL_VARIANT3(16, 16, 16, 1024, 4), GL_VARIANT3(16, 16, 16, 512, 2),
     GL_VARIANT3(8, 8, 8, 512, 8), GL_VARIANT3(16, 16, 8, 512, 4), GL_VARIANT3(16, 8, 8, 512, 4),
+    // 1024 points in 256-thread workgroups of two instances: twice the
+    // workgroups of the 512-thread form for cache-resident lengths
+    GL_VARIANT3(16, 8, 8, 256, 2), GL_VARIANT3(8, 8, 16, 256, 2),
 };
 
 const PassVariant *pass_variants(int *count) {
