@@ -40,6 +40,29 @@ def test_mixed_host_vs_oracle(gl, oracle):
         assert np.array_equal(x, y)
 
 
+def test_mixed_inverse_two_level_twiddles(gl, oracle):
+    """Grouped device batch, forward then inverse, with every output twiddle
+    from the two-level table: the inverse then folds 1/n into the first
+    pass's twiddle recurrence (two-sub-step and three-stage first passes)."""
+    import torch
+    sizes = mixed_sizes(24, 8, 14, 5)
+    note(f"mixed device inverse, two-level twiddles, {len(sizes)} transforms")
+    xs = [oracle.fill_random(n, 100 + i) for i, n in enumerate(sizes)]
+    ts = [to_torch(x) for x in xs]
+    prev = gl.set_twiddle_full_max(0)
+    try:
+        gl.ntt_batch_mixed(ts)
+        torch.cuda.synchronize()
+        for x, t, n in zip(xs, ts, sizes):
+            assert np.array_equal(from_torch(t), oracle.ntt(x)), n
+        gl.ntt_batch_mixed(ts, inverse=True)
+        torch.cuda.synchronize()
+        for x, t, n in zip(xs, ts, sizes):
+            assert np.array_equal(from_torch(t), x), n
+    finally:
+        gl.set_twiddle_full_max(prev)
+
+
 def test_mixed_device_alone_equals_in_mix(gl, oracle):
     import torch
     sizes = mixed_sizes(64, 10, 16, 2)
