@@ -8,6 +8,11 @@ sample).  Extra workloads: --workload batch (mixed-size batched NTTs,
 batched vs one-launch-sequence-per-transform, with p50/p99 latency),
 --workload commit (coset LDE + K12 Merkle commit), --workload serve (open-loop
 Poisson NTT/commit jobs through the continuous-batching server).
+
+The default line also carries the metric's second half, "mixed_batch": the
+1024-transform mixed-size batch (n = 2^k * f, k in [10, 20], f in
+{1, 3, 5, 15, 17, 257}) timed grouped, one launch sequence per transform and
+as a captured hipGraph, with p50/p99 batch latency.
 """
 import argparse
 import json
@@ -34,6 +39,8 @@ def parse():
                     help="ntt: skip the host-array (PCIe) timing, e.g. under a profiler")
     ap.add_argument("--batch-k", type=int, nargs=2, default=[10, 20], metavar=("LO", "HI"),
                     help="batch: transform lengths 2^k * f with k uniform in [LO, HI]")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="ntt: leave out the mixed-size batch sub-object")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
     return ap.parse_args()
@@ -66,37 +73,77 @@ def kernel_table(timings):
     return tab
 
 
+def host_cpu():
+    """(model name, logical CPUs this process may run on, threads used)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    # the host's CPU share for this GPU, where the environment states one
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(usable, share) if share > 0 else usable
+    return model, usable, max(1, threads)
+
+
 def cpu_baseline_ntt(n, seconds):
     """The oracle (C port of the reference's plan tree) on this host: the
-    top Cooley-Tukey level split over up to 16 threads (as the reference's
-    ntt-par splits it over rayon), then on one thread; whole 2^k transforms,
-    each timing bounded by about seconds/2 of wall time."""
+    top Cooley-Tukey level split over every usable host core (as the
+    reference's ntt-par splits it over rayon), then on one thread; whole 2^k
+    transforms timed with the reference's rule (bench/src/main.rs:61-95:
+    repeat until the accumulated time passes a budget, report the median),
+    each budget about seconds/2 of wall time."""
     if seconds <= 0:
         return None
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     x = oracle.fill_random(n, 1)
+    model, usable, threads = host_cpu()
 
-    def timed(threads, budget):
-        oracle.lib().or_set_threads(threads)
+    def timed(nthreads, budget):
+        oracle.lib().or_set_threads(nthreads)
         times = []
-        t_end = time.perf_counter() + budget
-        while True:
+        total = 0.0
+        while total < budget and len(times) < 50:
             t0 = time.perf_counter()
             oracle.ntt(x)
             times.append(time.perf_counter() - t0)
-            if time.perf_counter() >= t_end or len(times) >= 50:
-                break
+            total += times[-1]
         oracle.lib().or_set_threads(1)
         return n / statistics.median(times) / 1e9, len(times)
 
-    threads = max(1, min(16, os.cpu_count() or 1))
     par, k_par = timed(threads, seconds / 2)
     one, k_one = timed(1, seconds / 2)
     return {"value": round(par, 5), "unit": "Gelem/s", "cores": threads, "kind": "port",
             "sample": f"forward NTTs of 2^{n.bit_length() - 1} words, median of {k_par} "
                       f"({threads} threads) and of {k_one} (1 thread)",
-            "value_1core": round(one, 5)}
+            "value_1core": round(one, 5), "cpu_model": model, "host_logical_cpus": usable,
+            "note": "C port of the reference plan tree; reference serial speed at 2^24 is "
+                    "~0.010 Gelem/s on EPYC 7R13 and ~0.031 on M1 Max (bench/analysis.ipynb plots)"}
+
+
+def warm_up(fn, steps, torch, min_seconds=1.0, max_seconds=4.0):
+    """`steps` untimed calls, then more until at least min_seconds of wall
+    time have passed (GPU clocks settle under load), never past
+    max_seconds; synchronized at the end."""
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    while time.perf_counter() - t0 < min_seconds:
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        if time.perf_counter() - t0 > max_seconds:
+            break
 
 
 def bench_ntt(args, torch, gl):
@@ -109,9 +156,7 @@ def bench_ntt(args, torch, gl):
     y = x.clone()
     del y
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        gl.ntt(x)
-    torch.cuda.synchronize()
+    warm_up(lambda: gl.ntt(x), args.warmup, torch)
     # Timed region: exactly K steps, synchronized on both sides, bracketed by
     # two events on the launch stream.
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -183,11 +228,17 @@ def bench_ntt(args, torch, gl):
     copy_gbs = 16.0 * n / (c0.elapsed_time(c1) / 20 * 1e-3) / 1e9
     del buf
     plan = gl.plan_info(n)
-    # Dominant kernel: largest share of the timed region.  Each NTT pass reads
-    # and writes every word once, so a launch moves 16 n algorithmic bytes.
+    # Dominant kernel: largest share of the timed region.  The algorithmic
+    # bytes of a whole transform are 16 n (read and write each word once);
+    # a schedule of P pass launches is charged 16 n / P per launch.  The
+    # bytes a launch actually moves (16 n: every pass reads and writes every
+    # word) are reported beside it as launch_hbm_frac.
     kname = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else "none"
     avg_ms = kernels[kname]["total_ms"] / kernels[kname]["launches"] if kernels else ms
-    achieved = 16.0 * n / (avg_ms * 1e-3) / 1e9
+    npass = len(plan)
+    alg_per_launch = 16.0 * n / npass
+    achieved = alg_per_launch / (avg_ms * 1e-3) / 1e9
+    moved = 16.0 * n / (avg_ms * 1e-3) / 1e9
     gelem = n / (ms * 1e-3) / 1e9
     return {
         "metric": f"Gelem/s, 2^{args.log_n}-point forward Goldilocks NTT (HBM-resident)",
@@ -213,8 +264,11 @@ def bench_ntt(args, torch, gl):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_for(kname, f"ntt_2^{args.log_n}"),
-                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": 16 * n,
-                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches_per_transform": npass,
+                     "algorithmic_bytes_per_launch": int(alg_per_launch),
+                     "launch_bytes_moved": 16 * n, "launch_hbm_gbs": round(moved, 1),
+                     "launch_hbm_frac": round(moved / HBM_PEAK_GBS, 4),
+                     "copy_gbs": round(copy_gbs, 1), "launch_frac_of_copy": round(moved / copy_gbs, 4)},
     }
 
 
@@ -232,6 +286,12 @@ def main():
         raise SystemExit(gl.lib().gl_last_error().decode())
     if args.workload == "ntt":
         out = bench_ntt(args, torch, gl)
+        torch.cuda.empty_cache()
+        if not args.no_batch:
+            # the metric's second half: p50/p99 of the mixed-size batch
+            from goldilocks_amd.benchmarks import batch_summary
+            out["mixed_batch"] = batch_summary(torch, gl, args.batch_k)
+            torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline_ntt(1 << args.log_n, args.cpu_seconds)
     elif args.workload == "batch":
         from goldilocks_amd.benchmarks import bench_batch

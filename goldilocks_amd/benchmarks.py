@@ -9,66 +9,84 @@ def mixed_sizes(count, seed, lo=10, hi=20, factors=(1, 3, 5, 15, 17, 257)):
     return [(1 << rng.randint(lo, hi)) * rng.choice(factors) for _ in range(count)]
 
 
-def bench_batch(args, torch, gl):
-    """1024 forward transforms, n = 2^k * f (k uniform in [lo, hi], default
-    [10, 20], f from {1, 3, 5, 15, 17, 257}), device-resident: grouped
-    launches (the reported value), one launch sequence per transform, and the
-    batch captured once as a hipGraph and replayed; per-batch latency
-    p50/p99 over the repetitions."""
-    lo, hi = args.batch_k
-    sizes = mixed_sizes(1024, 7, lo, hi)
+def _timed_reps(torch, stream, fn, min_total_ms=1000.0, min_reps=5, max_reps=40):
+    """Per-repetition milliseconds (HIP events on `stream`) after one
+    untimed call, repeated until the accumulated time passes min_total_ms
+    (the reference's rule, bench/src/main.rs:61-95), within the rep bounds."""
+    fn()
+    torch.cuda.synchronize()
+    times = []
+    while len(times) < max_reps and (len(times) < min_reps or sum(times) < min_total_ms):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return times
+
+
+def _pct(times, q):
+    t = sorted(times)
+    return t[min(len(t) - 1, int(q * len(t)))]
+
+
+def batch_summary(torch, gl, batch_k=(10, 20), count=1024, seed=7):
+    """The mixed-size batch (count forward transforms, n = 2^k * f, k
+    uniform in batch_k, f in {1, 3, 5, 15, 17, 257}, device-resident), timed
+    three ways: grouped launches (gl_ntt_mixed_dev), one launch sequence per
+    transform, and the grouped schedule captured once as a hipGraph and
+    replayed.  Each: p50/p99 batch latency over the repetitions, Gelem/s at
+    p50."""
+    lo, hi = batch_k
+    sizes = mixed_sizes(count, seed, lo, hi)
     total = sum(sizes)
     bufs = [torch.empty(n, dtype=torch.int64, device="cuda") for n in sizes]
     for i, b in enumerate(bufs):
         gl.fill_random(b, i)
     stream = torch.cuda.current_stream()
-
-    def run(batched):
-        gl.ntt_batch_mixed(bufs, batched=batched)
-
     res = {}
-    reps = max(3, min(args.steps, 12))
-
-    def timed(key, fn):
-        fn()
-        torch.cuda.synchronize()
-        times = []
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            fn()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            times.append(e0.elapsed_time(e1))
-        times.sort()
-        med = times[len(times) // 2]
-        res[key] = {"ms_p50": round(med, 3), "ms_p99": round(times[min(len(times) - 1, int(0.99 * len(times)))], 3),
-                    "gelem_s": round(total / (med * 1e-3) / 1e9, 3)}
-
-    timed("batched", lambda: run(True))
-    timed("unbatched", lambda: run(False))
-    # the same batch planned once and replayed as one captured hipGraph
+    for key, fn in (("batched", lambda: gl.ntt_batch_mixed(bufs)),
+                    ("unbatched", lambda: gl.ntt_batch_mixed(bufs, batched=False))):
+        times = _timed_reps(torch, stream, fn)
+        res[key] = {"p50_ms": round(_pct(times, 0.5), 3), "p99_ms": round(_pct(times, 0.99), 3),
+                    "reps": len(times), "gelem_s": round(total / (_pct(times, 0.5) * 1e-3) / 1e9, 3)}
     plan = gl.BatchPlan(bufs)
-    timed("graph", lambda: plan.launch(stream))
+    times = _timed_reps(torch, stream, lambda: plan.launch(stream))
     plan.close()
+    res["graph"] = {"p50_ms": round(_pct(times, 0.5), 3), "p99_ms": round(_pct(times, 0.99), 3),
+                    "reps": len(times), "gelem_s": round(total / (_pct(times, 0.5) * 1e-3) / 1e9, 3)}
+    del bufs
+    return {
+        "workload": f"{count} forward NTTs, n = 2^k * f, k uniform in [{lo},{hi}], f in {{1,3,5,15,17,257}}, "
+                    f"seed {seed}, device-resident",
+        "elements": total,
+        "latency_ms": {"p50": res["batched"]["p50_ms"], "p99": res["batched"]["p99_ms"]},
+        "gelem_s": res["batched"]["gelem_s"],
+        "hbm_frac": round(16.0 * total / (res["batched"]["p50_ms"] * 1e-3) / 8e12, 4),
+        "batched": res["batched"], "unbatched": res["unbatched"], "graph": res["graph"],
+        "speedup_vs_unbatched": round(res["unbatched"]["p50_ms"] / res["batched"]["p50_ms"], 3),
+        "graph_speedup_vs_unbatched": round(res["unbatched"]["p50_ms"] / res["graph"]["p50_ms"], 3),
+    }
+
+
+def bench_batch(args, torch, gl):
+    """The mixed-size batch as its own line (batch_summary), reported value
+    = grouped launches."""
+    sm = batch_summary(torch, gl, args.batch_k)
     return {
         "metric": "Gelem/s, mixed-size batched NTT (1024 transforms); p50/p99 batch latency",
-        "value": res["batched"]["gelem_s"],
+        "value": sm["gelem_s"],
         "unit": "Gelem/s",
         "n_gpus": 1,
-        "steps": reps,
-        "ms_per_step": res["batched"]["ms_p50"],
+        "steps": sm["batched"]["reps"],
+        "ms_per_step": sm["batched"]["p50_ms"],
         "higher_is_better": True,
         "dtype": "u64",
         "data": "synthetic: seeded uniform [0,p) words, device-resident",
-        "config": {"workload": f"1024 forward NTTs, n = 2^k * f, k uniform in [{lo},{hi}], f in {{1,3,5,15,17,257}}",
-                   "elements": total, "bytes": 8 * total},
-        "latency_ms": {"p50": res["batched"]["ms_p50"], "p99": res["batched"]["ms_p99"]},
-        "batched": res["batched"],
-        "unbatched": res["unbatched"],
-        "graph": res["graph"],
-        "speedup_vs_unbatched": round(res["unbatched"]["ms_p50"] / res["batched"]["ms_p50"], 3),
-        "graph_speedup_vs_unbatched": round(res["unbatched"]["ms_p50"] / res["graph"]["ms_p50"], 3),
+        "config": {"workload": sm["workload"], "elements": sm["elements"], "bytes": 8 * sm["elements"]},
+        **{k: sm[k] for k in ("latency_ms", "hbm_frac", "batched", "unbatched", "graph", "speedup_vs_unbatched",
+                              "graph_speedup_vs_unbatched")},
     }
 
 

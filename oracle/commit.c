@@ -15,25 +15,38 @@
 #include <stdlib.h>
 #include <string.h>
 
+typedef struct {
+    const uint64_t *m;
+    uint64_t *lde;
+    size_t rows, cols, ext;
+} CommitCtx;
+
+/* One column: intt, zero-extend, coefficient r times 2717^r, ntt. */
+static void commit_column(void *arg, size_t c) {
+    const CommitCtx *cx = (const CommitCtx *)arg;
+    uint64_t *col = (uint64_t *)malloc(cx->ext * sizeof(uint64_t));
+    for (size_t r = 0; r < cx->rows; r++) col[r] = cx->m[r * cx->cols + c];
+    or_intt(col, cx->rows);
+    memset(col + cx->rows, 0, (cx->ext - cx->rows) * sizeof(uint64_t));
+    uint64_t g = 1;
+    for (size_t r = 0; r < cx->rows; r++) {
+        col[r] = or_mul(col[r], g);
+        g = or_mul(g, OR_GENERATOR);
+    }
+    or_ntt(col, cx->ext);
+    for (size_t r = 0; r < cx->ext; r++) cx->lde[r * cx->cols + c] = col[r];
+    free(col);
+}
+
 int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
               uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
     size_t ext = rows * (size_t)blowup;
     if (rows == 0 || cols == 0 || blowup == 0 || !or_is_divisor(rows) || !or_is_divisor(ext))
         return -1;
-    uint64_t *col = (uint64_t *)malloc(ext * sizeof(uint64_t));
-    for (size_t c = 0; c < cols; c++) {
-        for (size_t r = 0; r < rows; r++) col[r] = m[r * cols + c];
-        or_intt(col, rows);
-        memset(col + rows, 0, (ext - rows) * sizeof(uint64_t));
-        uint64_t g = 1;
-        for (size_t r = 0; r < rows; r++) {
-            col[r] = or_mul(col[r], g);
-            g = or_mul(g, OR_GENERATOR);
-        }
-        or_ntt(col, ext);
-        for (size_t r = 0; r < ext; r++) lde_out[r * cols + c] = col[r];
-    }
-    free(col);
+    CommitCtx cx = {m, lde_out, rows, cols, ext};
+    /* columns are independent: one per thread (each column's transforms
+     * then run serially inside the parallel loop) */
+    or_par_for(cols, ext * 64, commit_column, &cx);
     /* Field words are little-endian u64, so the byte view is the memory. */
     return or_merkle_tree((const uint8_t *)lde_out, ext * cols * 8, layers_out, cap, out_len);
 }

@@ -118,13 +118,28 @@ void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]) {
 /* mod.rs:33-35 */
 size_t or_next_layer_size(size_t len) { return (len + BLOCK_SIZE - 1) / BLOCK_SIZE * HASH_SIZE; }
 
-/* generic.rs:86-94: out[32i..32i+32] = K12(in[160i..min(160i+160,len)]). */
-void or_process_layer(const uint8_t *in, size_t len, uint8_t *out) {
-    for (size_t i = 0; i * BLOCK_SIZE < len; i++) {
-        size_t l = len - i * BLOCK_SIZE;
+typedef struct {
+    const uint8_t *in;
+    size_t len;
+    uint8_t *out;
+} LayerCtx;
+
+/* Blocks [64 i, 64 i + 64) of a layer. */
+static void layer_range(void *arg, size_t i) {
+    const LayerCtx *cx = (const LayerCtx *)arg;
+    for (size_t b = 64 * i; b < 64 * i + 64 && b * BLOCK_SIZE < cx->len; b++) {
+        size_t l = cx->len - b * BLOCK_SIZE;
         if (l > BLOCK_SIZE) l = BLOCK_SIZE;
-        or_k12(in + i * BLOCK_SIZE, l, out + i * HASH_SIZE, HASH_SIZE);
+        or_k12(cx->in + b * BLOCK_SIZE, l, cx->out + b * HASH_SIZE, HASH_SIZE);
     }
+}
+
+/* generic.rs:86-94: out[32i..32i+32] = K12(in[160i..min(160i+160,len)]).
+ * Blocks are independent, so ranges of 64 run in parallel (or_set_threads). */
+void or_process_layer(const uint8_t *in, size_t len, uint8_t *out) {
+    size_t blocks = (len + BLOCK_SIZE - 1) / BLOCK_SIZE;
+    LayerCtx cx = {in, len, out};
+    or_par_for((blocks + 63) / 64, 64 * 1024, layer_range, &cx);
 }
 
 size_t or_merkle_tree_size(size_t len) {

@@ -231,6 +231,12 @@ static void *run_range(void *arg) {
     return NULL;
 }
 
+static void par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), void *ctx);
+
+void or_par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), void *ctx) {
+    par_for(count, work_per_item, fn, ctx);
+}
+
 static void par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), void *ctx) {
     int threads = g_threads;
     if (threads > 256) threads = 256;

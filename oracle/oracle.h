@@ -66,8 +66,12 @@ int or_intt(uint64_t *v, size_t n);                 /* mod.rs:104-120 */
 int or_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse); /* strategy(n).ntt */
 /* X[k] of the forward transform, evaluated directly in O(n). */
 uint64_t or_eval_dft(const uint64_t *x, size_t n, uint64_t k);
-/* Host thread count used by or_ntt/or_ntt_batch outer loops (default 1). */
+/* Host thread count used by or_ntt/or_ntt_batch outer loops, the commit's
+ * column loop and each Merkle layer (default 1). */
 void or_set_threads(int threads);
+/* fn(ctx, i) for i < count on up to or_set_threads() threads (serial below
+ * 2^15 units of work, or when already inside a parallel loop). */
+void or_par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), void *ctx);
 
 /* ---- K12 / Merkle (pcs/src/k12/{generic,mod}.rs) ----------------------------
  * Pinned: 24-round permutation vs hashlib.sha3_256; XKCP KATs
