@@ -38,22 +38,21 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
     int rc = ensure_init();
     if (rc) return rc;
     if (inverse && n <= 1) return GL_OK;  // mod.rs:105-107
-    const Plan *plan = nullptr;
+    PlanRef plan;
     if ((rc = S0 > 1 ? get_plan_cols(n, &plan) : get_plan(n, &plan))) return rc;
     if (count == 0 || S0 == 0 || plan->passes.empty()) return GL_OK;
     if (!d) return set_error(GL_E_ARG, "null device pointer");
     const u64 bstride = n * S0;
-    size_t words = plan_tmp_words(*plan, count, bstride);
-    u64 *tmp = stream_scratch(s, words);
-    if (words && !tmp) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    return run_plan(*plan, d, tmp, S0, count, bstride, inverse, s);
+    const size_t words = plan_tmp_words(*plan, count, bstride);
+    DevBuf tmp(words * 8, s);  // this call's scratch, stream ordered
+    if (words && !tmp.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    return run_plan(*plan, d, tmp.as<u64>(), S0, count, bstride, inverse, s);
 }
 
-// 21 words of per-stream device scratch for a layer's padded partial block
-// (layers on one stream run in order, so they share it).
-// two 32-word halves: a layer reads one staged partial block and stages the
-// next layer's in the other
-u64 *k12_stage(hipStream_t s) { return stream_scratch(s, 64, 6); }
+// Device scratch of a tree or layer call: two 32-word halves, each holding
+// a layer's padded partial last block (a layer reads one and stages the next
+// layer's in the other).
+constexpr size_t kStageBytes = 64 * 8;
 
 int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap, size_t *out_len,
                     hipStream_t s) {
@@ -62,7 +61,9 @@ int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
     const uint8_t *layer = d_in;
     uint8_t *dst = d_out;
-    u64 *stage = k12_stage(s);
+    if (need == 0) return GL_OK;
+    DevBuf stage_buf(kStageBytes, s);
+    u64 *stage = stage_buf.as<u64>();
     if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
     int cur = 0;
     bool ready = false;
@@ -93,15 +94,36 @@ size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup) {
     return tree_size((u64)rows * blowup * cols * 8);
 }
 
+// The commit's two column plans (inverse over rows, forward over the
+// extension).
+static int commit_plans(size_t rows, size_t cols, unsigned blowup, PlanRef &prow, PlanRef &pext) {
+    const u64 ext = (u64)rows * blowup;
+    int rc;
+    if ((rc = cols > 1 ? get_plan_cols(rows, &prow) : get_plan(rows, &prow))) return rc;
+    return cols > 1 ? get_plan_cols(ext, &pext) : get_plan(ext, &pext);
+}
+
+int commit_prepare(size_t rows, size_t cols, unsigned blowup, Keep *keep) {
+    int rc = commit_check(rows, cols, blowup);
+    if (rc) return rc;
+    PlanRef prow, pext;
+    if ((rc = commit_plans(rows, cols, blowup, prow, pext))) return rc;
+    if ((rc = warm_plan(*prow, cols, keep)) || (rc = warm_plan(*pext, cols, keep))) return rc;
+    TableRef ct = coset_table(rows), sct = scaled_coset_table(rows);
+    PowTable g = power_table(2717, rows);
+    if (!ct || !sct || !g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    if (keep) keep->insert(keep->end(), {ct, sct, g.lo, g.hi});
+    return GL_OK;
+}
+
 // Enqueue the whole commit on stream s (no host synchronization):
 // coef (rows*cols words) and tmp (blowup*rows*cols words) are scratch.
 int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
                    u64 *coef, u64 *tmp, hipStream_t s) {
     const u64 ext = (u64)rows * blowup;
-    const Plan *prow = nullptr, *pext = nullptr;
+    PlanRef prow, pext;
     int rc;
-    if ((rc = cols > 1 ? get_plan_cols(rows, &prow) : get_plan(rows, &prow))) return rc;
-    if ((rc = cols > 1 ? get_plan_cols(ext, &pext) : get_plan(ext, &pext))) return rc;
+    if ((rc = commit_plans(rows, cols, blowup, prow, pext))) return rc;
     // extension input (coef shifted by 2717^j, zero rows j >= rows): when the
     // intt's last pass is a single kernel, the shift joins its 1/rows scale
     // (one product per output either way) and the extension's first pass
@@ -115,21 +137,22 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     // shift as well: two-sub-step ones
     const bool fold_scale = single && (first_kind == VK_TWO_STAGE || first_kind == VK_THREE_STAGE);
     const bool fold = fold_scale || first_kind == VK_TWO_STAGE;
-    const u64 *ct = fold && !fold_scale ? coset_table(rows) : nullptr;
-    const u64 *sct = fold_scale ? scaled_coset_table(rows) : nullptr;
+    // held until the passes reading them are enqueued
+    TableRef ct = fold && !fold_scale ? coset_table(rows) : nullptr;
+    TableRef sct = fold_scale ? scaled_coset_table(rows) : nullptr;
     if ((fold && !fold_scale && !ct) || (fold_scale && !sct))
         return set_error(GL_E_NOMEM, "twiddle table allocation failed");
     // intt of the columns, reading the input matrix in the first pass
     if (prow->passes.empty()) GL_HIP(hipMemcpyAsync(coef, d_m, rows * cols * 8, hipMemcpyDeviceToDevice, s));
-    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m, nullptr, 0, sct))) return rc;
+    if ((rc = run_plan(*prow, coef, tmp, cols, 1, rows * cols, true, s, true, d_m, nullptr, 0, sct.get()))) return rc;
     if (fold_scale) {
         if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, nullptr, rows))) return rc;
     } else if (fold) {
-        if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, ct, rows))) return rc;
+        if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s, true, coef, ct.get(), rows))) return rc;
     } else {
         PowTable g = power_table(2717, rows);
         if (!g.lo || !g.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-        launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo, g.hi, g.lo_bits, s);
+        launch_coset_extend(coef, d_lde, rows, cols, blowup, g.lo.get(), g.hi.get(), g.lo_bits, s);
         GL_HIP(hipGetLastError());
         if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s))) return rc;
     }
@@ -190,7 +213,7 @@ int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
     if (!v) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
     size_t bytes = n * count * 8;
-    DevBuf d(bytes);
+    DevBuf d(bytes, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     GL_HIP(hipMemcpyAsync(d.p, v, bytes, hipMemcpyHostToDevice, s));
     if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s))) return rc;
@@ -261,17 +284,17 @@ const char *gl_variant_name(uint32_t id) {
 static int plan_variant_ids(const Plan *plan, uint32_t *ids, size_t cap, size_t *npasses);
 
 int gl_plan_variants(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
-    const Plan *plan = nullptr;
+    PlanRef plan;
     int rc = get_plan(n, &plan);
     if (rc) return rc;
-    return plan_variant_ids(plan, ids, cap, npasses);
+    return plan_variant_ids(plan.get(), ids, cap, npasses);
 }
 
 int gl_plan_variants_cols(uint64_t n, uint32_t *ids, size_t cap, size_t *npasses) {
-    const Plan *plan = nullptr;
+    PlanRef plan;
     int rc = get_plan_cols(n, &plan);
     if (rc) return rc;
-    return plan_variant_ids(plan, ids, cap, npasses);
+    return plan_variant_ids(plan.get(), ids, cap, npasses);
 }
 
 static int plan_variant_ids(const Plan *plan, uint32_t *ids, size_t cap, size_t *npasses) {
@@ -309,7 +332,7 @@ int gl_plan_set_cols(uint64_t n, const uint32_t *variant_ids, size_t nvariants) 
 }
 
 int gl_plan_passes(uint64_t n, uint32_t *r1, uint32_t *r2, uint32_t *t, size_t cap, size_t *npasses) {
-    const Plan *plan = nullptr;
+    PlanRef plan;
     int rc = get_plan(n, &plan);
     if (rc) return rc;
     if (npasses) *npasses = plan->passes.size();
@@ -332,7 +355,7 @@ int gl_k12(const uint8_t *in, size_t len, uint8_t *out, size_t outlen) {
     if (len > 166 || outlen > 168) return set_error(GL_E_ARG, "k12: input > 166 bytes or output > 168 bytes");
     if ((len && !in) || (outlen && !out)) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
-    DevBuf d(256 + 256);
+    DevBuf d(256 + 256, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + 256;
     if (len) GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
@@ -347,9 +370,10 @@ int gl_k12_layer_dev(const uint8_t *d_in, size_t len, uint8_t *d_out, void *stre
     int rc = ensure_init();
     if (rc) return rc;
     if (len && (!d_in || !d_out)) return set_error(GL_E_ARG, "null device pointer");
-    u64 *stage = k12_stage(as_stream(stream));
-    if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    launch_k12_layer(d_in, len, d_out, stage, false, nullptr, as_stream(stream));
+    if (len == 0) return GL_OK;
+    DevBuf stage(kStageBytes, as_stream(stream));
+    if (!stage.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    launch_k12_layer(d_in, len, d_out, stage.as<u64>(), false, nullptr, as_stream(stream));
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -361,12 +385,12 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     if (!in || !out) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
     size_t olen = next_layer(len);
-    DevBuf d(len + olen + 256);
+    const size_t in_pad = (len + 255) / 256 * 256, out_pad = (olen + 255) / 256 * 256;
+    DevBuf d(in_pad + out_pad + kStageBytes, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
-    uint8_t *din = d.as<uint8_t>(), *dout = din + ((len + 255) / 256 * 256);
+    uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
+    u64 *stage = (u64 *)(dout + out_pad);
     GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
-    u64 *stage = k12_stage(s);
-    if (!stage) return set_error(GL_E_NOMEM, "scratch allocation failed");
     launch_k12_layer(din, len, dout, stage, false, nullptr, s);
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out, dout, olen, hipMemcpyDeviceToHost, s));
@@ -445,7 +469,7 @@ int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
     if (!in || !out) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
     size_t in_pad = (len + 255) / 256 * 256;
-    DevBuf d(in_pad + need);
+    DevBuf d(in_pad + need, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
     GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
@@ -460,7 +484,7 @@ int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_no
     if (rc) return rc;
     if (!challenge32 || !nonce) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
-    DevBuf d(256);
+    DevBuf d(256, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     unsigned long long *best = (unsigned long long *)d.p;
     uint8_t *dch = (uint8_t *)d.p + 64;
@@ -499,7 +523,7 @@ int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
     if (!gl_is_divisor(n)) return set_error(GL_E_SIZE, "convolve: length does not divide p - 1");
     if (!a || !b_ntt) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
-    DevBuf d(2 * n * 8);
+    DevBuf d(2 * n * 8, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     u64 *da = d.as<u64>(), *db = da + n;
     GL_HIP(hipMemcpyAsync(da, a, n * 8, hipMemcpyHostToDevice, s));
@@ -523,14 +547,11 @@ int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup
     if (out_len) *out_len = need;
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
     if (!d_m || !d_lde || !d_layers) return set_error(GL_E_ARG, "null device pointer");
-    DevBuf coef(rows * cols * 8);
-    DevBuf tmp(ext * cols * 8);
+    // scratch returns to the pool stream ordered: no wait needed
+    DevBuf coef(rows * cols * 8, s);
+    DevBuf tmp(ext * cols * 8, s);
     if (!coef.p || !tmp.p) return set_error(GL_E_NOMEM, "commit scratch allocation failed");
-    if ((rc = commit_enqueue(d_m, rows, cols, blowup, d_lde, d_layers, coef.as<u64>(), tmp.as<u64>(), s)))
-        return rc;
-    // coef/tmp return to the pool on exit: finish the work first.
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return commit_enqueue(d_m, rows, cols, blowup, d_lde, d_layers, coef.as<u64>(), tmp.as<u64>(), s);
 }
 
 int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
@@ -544,7 +565,7 @@ int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
     if (!m || !lde_out || !layers_out) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
-    DevBuf dm(rows * cols * 8), dl(ext * cols * 8), dt(need + 256);
+    DevBuf dm(rows * cols * 8, s), dl(ext * cols * 8, s), dt(need + 256, s);
     if (!dm.p || !dl.p || !dt.p) return set_error(GL_E_NOMEM, "device allocation failed");
     GL_HIP(hipMemcpyAsync(dm.p, m, rows * cols * 8, hipMemcpyHostToDevice, s));
     if ((rc = gl_commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need,

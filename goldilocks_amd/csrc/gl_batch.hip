@@ -32,7 +32,8 @@ struct Group {
 };
 
 struct Schedule {
-    std::vector<const Plan *> plans;
+    std::vector<PlanRef> plans;
+    Keep keep;                                // tables the grouped launches read
     std::vector<size_t> alone;                // run as single transforms (gl_ntt_dev)
     std::vector<std::vector<size_t>> chunks;  // grouped level by level
     std::vector<u64> tmp_off;                 // scratch offset; restarts in every chunk
@@ -63,7 +64,7 @@ u64 chunk_words() {
 }
 
 int plan_batch(uint64_t *const *d, const size_t *n, size_t k, bool inverse, Schedule &sc) {
-    sc.plans.assign(k, nullptr);
+    sc.plans.assign(k, PlanRef());
     sc.tmp_off.assign(k, 0);
     sc.tmp_words = 0;
     const u64 cw = chunk_words();
@@ -73,7 +74,7 @@ int plan_batch(uint64_t *const *d, const size_t *n, size_t k, bool inverse, Sche
         if (inverse && n[i] <= 1) continue;
         if ((rc = get_plan(n[i], &sc.plans[i]))) return rc;
         if (sc.plans[i]->passes.empty()) {
-            sc.plans[i] = nullptr;
+            sc.plans[i].reset();
             continue;
         }
         if (!d[i]) return set_error(GL_E_ARG, "null device pointer");
@@ -103,7 +104,7 @@ int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Sc
         std::vector<PassCursor> cur;
         cur.reserve(chunk.size());
         for (size_t i : chunk) {
-            PassCursor c{sc.plans[i], d[i], tmp ? tmp + sc.tmp_off[i] : nullptr, 1, 1, n[i], inverse};
+            PassCursor c{sc.plans[i].get(), d[i], tmp ? tmp + sc.tmp_off[i] : nullptr, 1, 1, n[i], inverse};
             c.begin();
             levels = std::max(levels, sc.plans[i]->passes.size());
             cur.push_back(c);
@@ -136,6 +137,7 @@ int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Sc
                 a = b;
             }
         }
+        for (PassCursor &c : cur) sc.keep.insert(sc.keep.end(), c.keep.begin(), c.keep.end());
     }
     return GL_OK;
 }
@@ -182,20 +184,19 @@ extern "C" int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, i
     hipStream_t s = (hipStream_t)stream;
     Schedule sc;
     if ((rc = plan_batch(d, n, k, inverse != 0, sc))) return rc;
-    // stand-alone transforms first: they may grow this stream's scratch, which
-    // the grouped launches below then reuse
+    // stand-alone transforms first (each takes its own scratch)
     for (size_t i : sc.alone)
         if ((rc = gl_ntt_dev(d[i], n[i], 1, inverse, stream))) return rc;
-    u64 *tmp = stream_scratch(s, sc.tmp_words, 0);
-    if (sc.tmp_words && !tmp) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    if ((rc = build_groups(d, n, inverse != 0, tmp, sc))) return rc;
+    DevBuf tmp(sc.tmp_words * 8, s);
+    if (sc.tmp_words && !tmp.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
+    if ((rc = build_groups(d, n, inverse != 0, tmp.as<u64>(), sc))) return rc;
     if (sc.groups.empty()) return GL_OK;
-    u64 *meta = stream_scratch(s, sc.meta_bytes() / 8 + 64, 1);
-    if (!meta) return set_error(GL_E_NOMEM, "descriptor allocation failed");
+    DevBuf meta(sc.meta_bytes() + 512, s);
+    if (!meta.p) return set_error(GL_E_NOMEM, "descriptor allocation failed");
     std::vector<unsigned char> host;
     pack_meta(sc, host);
-    GL_HIP(hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s));
-    if ((rc = launch_groups(sc, meta, s, true))) return rc;
+    GL_HIP(hipMemcpyAsync(meta.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    if ((rc = launch_groups(sc, meta.p, s, true))) return rc;
     // `host` is released on return: wait until the copy has consumed it.
     GL_HIP(hipStreamSynchronize(s));
     return GL_OK;
@@ -215,7 +216,7 @@ extern "C" int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k,
         total += (n[i] + 31) / 32 * 32;
     }
     hipStream_t s = thread_stream();
-    DevBuf d(total * 8 + 8);
+    DevBuf d(total * 8 + 8, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     std::vector<uint64_t *> ptrs(k);
     for (size_t i = 0; i < k; i++) {
@@ -234,6 +235,7 @@ extern "C" int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k,
 struct gl_batch_plan {
     void *meta = nullptr;  // descriptors + block starts (device, owned)
     void *tmp = nullptr;   // ping-pong scratch (device, owned)
+    Keep keep;             // twiddle tables the captured launches read
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -281,18 +283,12 @@ extern "C" int gl_batch_plan_create(uint64_t *const *d, const size_t *n, size_t 
     // build the stand-alone transforms' twiddle tables now: table setup
     // synchronizes the device, which a capturing thread may not do
     for (size_t i : sc.alone) {
-        PassCursor c{sc.plans[i], d[i], (u64 *)p->tmp, 1, 1, n[i], inverse != 0};
-        c.begin();
-        while (!c.done() && !rc) {
-            PassArgs pa;
-            unsigned grid = 0;
-            rc = c.next(pa, grid);
-        }
-        if (rc) {
+        if ((rc = warm_plan(*sc.plans[i], 1, &p->keep))) {
             gl_batch_plan_destroy(p);
             return rc;
         }
     }
+    p->keep.insert(p->keep.end(), sc.keep.begin(), sc.keep.end());
     std::vector<unsigned char> host;
     pack_meta(sc, host);
     hipStream_t cs = nullptr;

@@ -35,6 +35,7 @@ static int g_init_rc = GL_OK;
 static std::string g_init_msg;
 
 static void load_tune_file();
+static std::once_flag g_tune_once;
 
 int ensure_init() {
     std::call_once(g_init_once, [] {
@@ -58,38 +59,70 @@ int ensure_init() {
             g_init_msg = std::string("library is built for gfx950, device is ") + prop.gcnArchName;
             return;
         }
-        load_tune_file();
+        std::call_once(g_tune_once, load_tune_file);
     });
     if (g_init_rc != GL_OK) return set_error(g_init_rc, g_init_msg);
     return GL_OK;
 }
 
 // ---------------------------------------------------------------- pool
-static std::mutex g_pool_mu;
-static std::multimap<size_t, void *> g_pool_free;
-static std::map<void *, size_t> g_pool_size;
+// Free blocks remember the stream that last used them and an event recorded
+// there at the free.  A block is handed out again at once for the same
+// stream (stream order runs the earlier user first), to any other stream
+// once its event has completed.  Blocks freed on a capturing stream become
+// part of a graph and are never reused.
+namespace {
+struct FreeBlock {
+    void *p;
+    hipStream_t stream;
+    hipEvent_t ev;  // nullptr: no pending use
+};
+std::mutex g_pool_mu;
+std::multimap<size_t, FreeBlock> g_pool_free;
+std::map<void *, size_t> g_pool_size;
+std::vector<hipEvent_t> g_pool_events;
 
-static size_t round_size(size_t b) {
+size_t round_size(size_t b) {
     const size_t MB = 1 << 20;
     if (b >= MB) return (b + MB - 1) / MB * MB;
     return (b + 4095) / 4096 * 4096;
 }
 
-void *pool_alloc(size_t bytes) {
-    size_t sz = round_size(bytes);
+// caller holds g_pool_mu
+bool block_ready(const FreeBlock &fb, hipStream_t s) {
+    if (!fb.ev || fb.stream == s) return true;
+    hipError_t q = hipEventQuery(fb.ev);
+    if (q == hipSuccess) return true;
+    if (q != hipErrorNotReady) (void)hipGetLastError();
+    return false;
+}
+
+// caller holds g_pool_mu
+void recycle_event(hipEvent_t e) {
+    if (e) g_pool_events.push_back(e);
+}
+}  // namespace
+
+void *pool_alloc(size_t bytes, hipStream_t s) {
+    const size_t sz = round_size(bytes);
     {
         std::lock_guard<std::mutex> lk(g_pool_mu);
-        auto it = g_pool_free.lower_bound(sz);
-        if (it != g_pool_free.end() && it->first <= sz + sz / 4) {
-            void *p = it->second;
-            g_pool_free.erase(it);
-            return p;
+        auto lo = g_pool_free.lower_bound(sz), hi = g_pool_free.upper_bound(sz + sz / 4);
+        for (int pass = 0; pass < 2; pass++) {
+            for (auto it = lo; it != hi; ++it) {
+                const bool ok = pass == 0 ? (!it->second.ev || it->second.stream == s) : block_ready(it->second, s);
+                if (!ok) continue;
+                void *p = it->second.p;
+                recycle_event(it->second.ev);
+                g_pool_free.erase(it);
+                return p;
+            }
         }
     }
     void *p = nullptr;
     if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
-        pool_trim();
+        pool_release_idle();
         if (hipMalloc(&p, sz) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
@@ -100,23 +133,56 @@ void *pool_alloc(size_t bytes) {
     return p;
 }
 
-void pool_free(void *p) {
+void pool_free(void *p, hipStream_t s) {
     if (!p) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        cs = hipStreamCaptureStatusNone;
+    }
+    if (cs != hipStreamCaptureStatusNone) return;  // owned by the captured graph from now on
     std::lock_guard<std::mutex> lk(g_pool_mu);
     auto it = g_pool_size.find(p);
     if (it == g_pool_size.end()) return;
-    g_pool_free.emplace(it->second, p);
+    hipEvent_t e = nullptr;
+    if (!g_pool_events.empty()) {
+        e = g_pool_events.back();
+        g_pool_events.pop_back();
+    } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+    }
+    if (e && hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(e);
+        e = nullptr;
+        // no event: only safe on the same stream; wait for it instead
+        (void)hipStreamSynchronize(s);
+    }
+    g_pool_free.emplace(it->second, FreeBlock{p, s, e});
 }
 
-void pool_trim() {
+void pool_release_idle() {
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    if (g_pool_free.empty()) return;
-    (void)hipDeviceSynchronize();
-    for (auto &kv : g_pool_free) {
-        g_pool_size.erase(kv.second);
-        (void)hipFree(kv.second);
+    for (auto it = g_pool_free.begin(); it != g_pool_free.end();) {
+        FreeBlock &fb = it->second;
+        bool done = true;
+        if (fb.ev) {
+            hipError_t q = hipEventQuery(fb.ev);
+            if (q != hipSuccess) {
+                if (q != hipErrorNotReady) (void)hipGetLastError();
+                done = false;
+            }
+        }
+        if (!done) {
+            ++it;
+            continue;
+        }
+        recycle_event(fb.ev);
+        g_pool_size.erase(fb.p);
+        (void)hipFree(fb.p);
+        it = g_pool_free.erase(it);
     }
-    g_pool_free.clear();
 }
 
 struct ThreadStream {
@@ -137,44 +203,44 @@ hipStream_t thread_stream() {
     return t_stream.s;
 }
 
-static std::mutex g_scratch_mu;
-static std::map<std::pair<hipStream_t, int>, std::pair<void *, size_t>> g_scratch;
-
-u64 *stream_scratch(hipStream_t s, size_t words, int slot) {
-    if (words == 0) return nullptr;
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    auto &e = g_scratch[std::make_pair(s, slot)];
-    if (e.second >= words * 8) return (u64 *)e.first;
-    if (e.first) {
-        (void)hipStreamSynchronize(s);
-        pool_free(e.first);
-        e.first = nullptr;
-        e.second = 0;
-    }
-    void *p = pool_alloc(words * 8);
-    if (!p) return nullptr;
-    e.first = p;
-    e.second = words * 8;
-    return (u64 *)p;
-}
-
-static void release_scratch() {
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    for (auto &kv : g_scratch)
-        if (kv.second.first) pool_free(kv.second.first);
-    g_scratch.clear();
-}
-
 // ---------------------------------------------------------------- twiddles
+// Retired tables (last reference dropped) wait here until clear_caches()
+// frees them after a device synchronization.
+static std::mutex g_grave_mu;
+static std::vector<void *> g_grave;
+
+static TableRef adopt(void *d) {
+    return TableRef((const u64 *)d, [](const u64 *p) {
+        std::lock_guard<std::mutex> lk(g_grave_mu);
+        g_grave.push_back((void *)p);
+    });
+}
+
+// Table generators run on a private stream, so building a table waits for
+// that stream only, never for work the callers have in flight.
+static hipStream_t table_stream() {
+    static std::once_flag once;
+    static hipStream_t s = nullptr;
+    std::call_once(once, [] {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            s = nullptr;
+        }
+    });
+    return s;
+}
+
 static std::mutex g_tw_mu;
 static std::map<std::pair<u64, u64>, PowTable> g_pow_tables;  // (base, count)
-static std::map<u64, const u64 *> g_inner_tables;
-static std::vector<void *> g_tw_allocs;
-static const u64 *g_rader257 = nullptr;
+static std::map<u64, TableRef> g_inner_tables;
+static std::map<std::pair<u64, u64>, TableRef> g_inner_matrices;
+static std::map<std::pair<u64, u64>, TableRef> g_full_tables;
+static std::map<u64, TableRef> g_coset_tables, g_scaled_coset_tables;
+static TableRef g_rader257;
 
 static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
-static const u64 *upload(const std::vector<u64> &h) {
+static TableRef upload(const std::vector<u64> &h) {
     void *d = nullptr;
     if (hipMalloc(&d, h.size() * 8) != hipSuccess) {
         (void)hipGetLastError();
@@ -185,8 +251,26 @@ static const u64 *upload(const std::vector<u64> &h) {
         (void)hipFree(d);
         return nullptr;
     }
-    g_tw_allocs.push_back(d);
-    return (const u64 *)d;
+    return adopt(d);
+}
+
+// Allocate `words`, run `gen(ptr, stream)` on the table stream and wait
+// for it; nullptr on failure.
+template <class F>
+static TableRef generate(u64 words, F gen) {
+    hipStream_t ts = table_stream();
+    void *d = nullptr;
+    if (!ts || hipMalloc(&d, words * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    gen((u64 *)d, ts);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ts) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d);
+        return nullptr;
+    }
+    return adopt(d);
 }
 
 PowTable power_table(u64 base, u64 count) {
@@ -218,31 +302,18 @@ PowTable power_table(u64 base, u64 count) {
 
 PowTable root_table(u64 M) { return power_table(root(M), M); }
 
-static std::map<std::pair<u64, u64>, const u64 *> g_full_tables;
-
-// Output-order twiddles of a pass with radix R on sub-problems of length M:
-// entry k + R*m = w_M^(m*k) (Montgomery form), M words.
-const u64 *twiddle_matrix(u64 M, u64 R) {
+TableRef twiddle_matrix(u64 M, u64 R) {
     PowTable t = root_table(M);
     if (!t.lo || !t.hi) return nullptr;
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto key = std::make_pair(M, R);
     auto it = g_full_tables.find(key);
     if (it != g_full_tables.end()) return it->second;
-    void *d = nullptr;
-    if (hipMalloc(&d, M * 8) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    launch_twiddle_matrix((u64 *)d, M, R, t.lo, t.hi, t.lo_bits, nullptr);
-    if (hipDeviceSynchronize() != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(d);
-        return nullptr;
-    }
-    g_tw_allocs.push_back(d);
-    g_full_tables[key] = (const u64 *)d;
-    return (const u64 *)d;
+    TableRef d = generate(M, [&](u64 *out, hipStream_t ts) {
+        launch_twiddle_matrix(out, M, R, t.lo.get(), t.hi.get(), t.lo_bits, ts);
+    });
+    if (d) g_full_tables[key] = d;
+    return d;
 }
 
 // A full twiddle matrix costs M words and one read per output; it replaces
@@ -260,9 +331,7 @@ static std::atomic<u64> &full_table_max_ref() {
 static u64 full_table_max() { return full_table_max_ref().load(); }
 u64 set_full_table_max(u64 words) { return full_table_max_ref().exchange(words); }
 
-const PassVariant *rader65537_variant();
-
-const u64 *inner_table(u64 R) {
+TableRef inner_table(u64 R) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_inner_tables.find(R);
     if (it != g_inner_tables.end()) return it->second;
@@ -272,64 +341,38 @@ const u64 *inner_table(u64 R) {
         t[e] = to_mont(v);
         v = mul(v, w);
     }
-    const u64 *d = upload(t);
+    TableRef d = upload(t);
     if (d) g_inner_tables[R] = d;
     return d;
 }
 
-// Montgomery g^j for j < rows, g = 2717 (the commit's coset shift, applied
-// by the first pass of the extension).  rows words, cached per rows.
-static std::map<u64, const u64 *> g_coset_tables;
-const u64 *coset_table(u64 rows) {
+TableRef coset_table(u64 rows) {
     PowTable t = power_table(2717, rows);
     if (!t.lo || !t.hi) return nullptr;
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_coset_tables.find(rows);
     if (it != g_coset_tables.end()) return it->second;
-    void *d = nullptr;
-    if (hipMalloc(&d, rows * 8) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    launch_power_series((u64 *)d, rows, t.lo, t.hi, t.lo_bits, nullptr);
-    if (hipDeviceSynchronize() != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(d);
-        return nullptr;
-    }
-    g_tw_allocs.push_back(d);
-    g_coset_tables[rows] = (const u64 *)d;
-    return (const u64 *)d;
+    TableRef d = generate(rows, [&](u64 *out, hipStream_t ts) {
+        launch_power_series(out, rows, t.lo.get(), t.hi.get(), t.lo_bits, ts);
+    });
+    if (d) g_coset_tables[rows] = d;
+    return d;
 }
 
-static std::map<u64, const u64 *> g_scaled_coset_tables;
-const u64 *scaled_coset_table(u64 rows) {
-    const u64 *ct = coset_table(rows);
+TableRef scaled_coset_table(u64 rows) {
+    TableRef ct = coset_table(rows);
     if (!ct) return nullptr;
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_scaled_coset_tables.find(rows);
     if (it != g_scaled_coset_tables.end()) return it->second;
-    void *d = nullptr;
-    if (hipMalloc(&d, rows * 8) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    launch_scale_table((u64 *)d, ct, rows, to_mont(inv(rows % P)), nullptr);
-    if (hipDeviceSynchronize() != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(d);
-        return nullptr;
-    }
-    g_tw_allocs.push_back(d);
-    g_scaled_coset_tables[rows] = (const u64 *)d;
-    return (const u64 *)d;
+    TableRef d = generate(rows, [&](u64 *out, hipStream_t ts) {
+        launch_scale_table(out, ct.get(), rows, to_mont(inv(rows % P)), ts);
+    });
+    if (d) g_scaled_coset_tables[rows] = d;
+    return d;
 }
 
-// Inner twiddles of a two-sub-step pass R = R1 x R2, ordered so that one
-// thread's R1 - 1 multipliers are consecutive: entry rb*R1 + ka =
-// w_R^(ka*rb) (Montgomery form).
-static std::map<std::pair<u64, u64>, const u64 *> g_inner_matrices;
-const u64 *inner_matrix(u64 R1, u64 R2) {
+TableRef inner_matrix(u64 R1, u64 R2) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto key = std::make_pair(R1, R2);
     auto it = g_inner_matrices.find(key);
@@ -343,15 +386,16 @@ const u64 *inner_matrix(u64 R1, u64 R2) {
     }
     for (u64 rb = 0; rb < R2; rb++)
         for (u64 ka = 0; ka < R1; ka++) t[rb * R1 + ka] = pw[(ka * rb) % R];
-    const u64 *d = upload(t);
+    TableRef d = upload(t);
     if (d) g_inner_matrices[key] = d;
     return d;
 }
 
 // Tables for the radix-257 Rader pass (rader.rs:19-57), Montgomery form:
 // [0, 256) w_256^(k1*j) at entry j*16 + k1 (a thread's inner twiddles are
-// consecutive), [256, 512) the convolution spectrum DFT_256(w_257^(3^q)/256).
-const u64 *rader257_table() {
+// consecutive), [256, 512) the convolution spectrum DFT_256(w_257^(3^q)/256),
+// [512, 768) the conjugate inner twiddles of the second (inverse) DFT_256.
+TableRef rader257_table() {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     if (g_rader257) return g_rader257;
     std::vector<u64> pw(256), t(768);
@@ -388,8 +432,7 @@ static const PassVariant kRader65537 = {65537, 1, 1, 256, 0, nullptr, nullptr, V
 const PassVariant *rader65537_variant() { return &kRader65537; }
 
 struct BigRaderTables {
-    const u32 *permk = nullptr, *permi = nullptr;
-    const u64 *spec = nullptr;
+    TableRef permk, permi, spec;  // permk/permi hold u32 entries
 };
 static BigRaderTables g_big_rader;
 
@@ -413,54 +456,53 @@ static int big_rader_tables(BigRaderTables &out) {
         a = a * 21846 % P2;
         b = b * 3 % P2;
     }
-    void *dpk = nullptr, *dpi = nullptr, *dspec = nullptr, *dtmp = nullptr;
-    if (hipMalloc(&dpk, M2 * 4) != hipSuccess || hipMalloc(&dpi, M2 * 4) != hipSuccess ||
-        hipMalloc(&dspec, M2 * 8) != hipSuccess || hipMalloc(&dtmp, M2 * 8) != hipSuccess) {
-        (void)hipGetLastError();
-        return set_error(GL_E_NOMEM, "radix-65537 table allocation failed");
-    }
-    GL_HIP(hipMemcpy(dpk, pk.data(), M2 * 4, hipMemcpyHostToDevice));
-    GL_HIP(hipMemcpy(dpi, pi.data(), M2 * 4, hipMemcpyHostToDevice));
-    GL_HIP(hipMemcpy(dspec, c.data(), M2 * 8, hipMemcpyHostToDevice));
-    // spectrum = DFT_65536(c) (linear, so Montgomery form is preserved)
-    const Plan *p65536 = nullptr;
+    std::vector<u64> pk64(M2 / 2), pi64(M2 / 2);
+    std::memcpy(pk64.data(), pk.data(), M2 * 4);
+    std::memcpy(pi64.data(), pi.data(), M2 * 4);
+    BigRaderTables t;
+    t.permk = upload(pk64);
+    t.permi = upload(pi64);
+    t.spec = upload(c);
+    if (!t.permk || !t.permi || !t.spec) return set_error(GL_E_NOMEM, "radix-65537 table allocation failed");
+    // spectrum = DFT_65536(c) (linear, so Montgomery form is preserved),
+    // computed on the table stream
+    PlanRef p65536;
     int rc = get_plan(M2, &p65536);
     if (rc) return rc;
-    if ((rc = run_plan(*p65536, (u64 *)dspec, (u64 *)dtmp, 1, 1, M2, false, nullptr))) return rc;
-    GL_HIP(hipDeviceSynchronize());
-    (void)hipFree(dtmp);
-    std::lock_guard<std::mutex> lk(g_tw_mu);
-    if (!g_big_rader.spec) {
-        g_big_rader.permk = (const u32 *)dpk;
-        g_big_rader.permi = (const u32 *)dpi;
-        g_big_rader.spec = (const u64 *)dspec;
-        g_tw_allocs.push_back(dpk);
-        g_tw_allocs.push_back(dpi);
-        g_tw_allocs.push_back(dspec);
+    hipStream_t ts = table_stream();
+    if (!ts) return set_error(GL_E_DEVICE, "table stream creation failed");
+    {
+        DevBuf tmp(M2 * 8, ts);
+        if (!tmp.p) return set_error(GL_E_NOMEM, "radix-65537 table allocation failed");
+        if ((rc = run_plan(*p65536, (u64 *)t.spec.get(), tmp.as<u64>(), 1, 1, M2, false, ts))) return rc;
     }
+    GL_HIP(hipStreamSynchronize(ts));
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (!g_big_rader.spec) g_big_rader = t;
     out = g_big_rader;
     return GL_OK;
 }
 
-static int run_rader_big(const PassArgs &a, hipStream_t stream) {
+static int run_rader_big(const PassArgs &a, hipStream_t stream, Keep &keep) {
     BigRaderTables t;
     int rc = big_rader_tables(t);
     if (rc) return rc;
-    const Plan *p65536 = nullptr;
+    keep.push_back(t.permk);
+    keep.push_back(t.permi);
+    keep.push_back(t.spec);
+    PlanRef p65536;
     if ((rc = get_plan(65536, &p65536))) return rc;
     const u64 ni = a.total_inst, words = ni * 65536;
-    u64 *C = stream_scratch(stream, words, 2);
-    u64 *Ct = stream_scratch(stream, words, 3);
-    u64 *x0 = stream_scratch(stream, ni, 4);
-    if (!C || !Ct || !x0) return set_error(GL_E_NOMEM, "radix-65537 scratch allocation failed");
-    launch_rader_big_gather(a, t.permk, C, x0, stream);
+    DevBuf C(words * 8, stream), Ct(words * 8, stream), x0(ni * 8, stream);
+    if (!C.p || !Ct.p || !x0.p) return set_error(GL_E_NOMEM, "radix-65537 scratch allocation failed");
+    launch_rader_big_gather(a, (const u32 *)t.permk.get(), C.as<u64>(), x0.as<u64>(), stream);
     GL_HIP(hipGetLastError());
-    if ((rc = run_plan(*p65536, C, Ct, 1, ni, 65536, false, stream))) return rc;
-    launch_rader_big_pointwise(C, x0, t.spec, ni, stream);
+    if ((rc = run_plan(*p65536, C.as<u64>(), Ct.as<u64>(), 1, ni, 65536, false, stream))) return rc;
+    launch_rader_big_pointwise(C.as<u64>(), x0.as<u64>(), t.spec.get(), ni, stream);
     GL_HIP(hipGetLastError());
     // reverse C[1..] and transform again (unscaled: the spectrum holds 1/65536)
-    if ((rc = run_plan(*p65536, C, Ct, 1, ni, 65536, true, stream, false))) return rc;
-    launch_rader_big_scatter(a, t.permi, C, x0, stream);
+    if ((rc = run_plan(*p65536, C.as<u64>(), Ct.as<u64>(), 1, ni, 65536, true, stream, false))) return rc;
+    launch_rader_big_scatter(a, (const u32 *)t.permi.get(), C.as<u64>(), x0.as<u64>(), stream);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -472,9 +514,12 @@ bool plan_is_composite(const Plan &plan) {
 }
 
 // ---------------------------------------------------------------- planner
+// Built plans (a cache), and the schedules that decide them: explicit ones
+// (set_plan) first, then the tune file's, then the planner's default.
+// Index 0: contiguous transforms, 1: column transforms (S0 > 1).
 static std::mutex g_plan_mu;
-static std::map<u64, std::unique_ptr<Plan>> g_plans;
-static std::map<u64, std::unique_ptr<Plan>> g_plans_cols;  // column transforms (S0 > 1)
+static std::map<u64, PlanRef> g_plans[2];
+static std::map<u64, std::vector<u32>> g_explicit[2], g_tuned[2];
 
 static const PassVariant *find_variant(int R1, int R2) {
     int cnt = 0;
@@ -602,62 +647,64 @@ static bool build_plan(u64 n, Plan &plan, bool cols = false) {
     return true;
 }
 
-int get_plan(u64 n, const Plan **out) {
+// Pass list from variant ids (index == compiled variant count: the
+// radix-257 Rader pass); false when an id is unknown or the radices do not
+// multiply to n.
+static bool plan_from_ids(u64 n, const std::vector<u32> &ids, Plan &plan) {
+    int cnt = 0;
+    const PassVariant *vs = pass_variants(&cnt);
+    plan.n = n;
+    plan.passes.clear();
+    u64 prod = 1;
+    for (u32 id : ids) {
+        const PassVariant *v = nullptr;
+        if (id < (u32)cnt) v = &vs[id];
+        else if (id == (u32)cnt) v = rader257_variant();
+        if (!v) return false;
+        u64 R = (u64)v->R1 * (u64)v->R2;
+        plan.passes.push_back({v, R});
+        prod *= R;
+    }
+    return prod == n;
+}
+
+static int get_plan_impl(u64 n, PlanRef *out, int cols) {
     if (n == 0 || (P - 1) % n != 0)
         return set_error(GL_E_SIZE, std::to_string(n) +
                                         " is not a supported NTT size (does not divide the multiplicative order)");
+    std::call_once(g_tune_once, load_tune_file);
     std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans.find(n);
-    if (it == g_plans.end()) {
-        std::unique_ptr<Plan> p(new Plan());
-        if (!build_plan(n, *p))
-            return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
-        it = g_plans.emplace(n, std::move(p)).first;
+    auto it = g_plans[cols].find(n);
+    if (it == g_plans[cols].end()) {
+        std::shared_ptr<Plan> p(new Plan());
+        auto ex = g_explicit[cols].find(n);
+        auto tu = g_tuned[cols].find(n);
+        bool ok;
+        if (ex != g_explicit[cols].end()) ok = plan_from_ids(n, ex->second, *p);
+        else if (tu != g_tuned[cols].end()) ok = plan_from_ids(n, tu->second, *p);
+        else ok = build_plan(n, *p, cols != 0);
+        if (!ok) return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
+        it = g_plans[cols].emplace(n, std::move(p)).first;
     }
-    *out = it->second.get();
+    if (out) *out = it->second;
     return GL_OK;
 }
 
-int get_plan_cols(u64 n, const Plan **out) {
-    if (n == 0 || (P - 1) % n != 0)
-        return set_error(GL_E_SIZE, std::to_string(n) +
-                                        " is not a supported NTT size (does not divide the multiplicative order)");
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans_cols.find(n);
-    if (it == g_plans_cols.end()) {
-        std::unique_ptr<Plan> p(new Plan());
-        if (!build_plan(n, *p, true))
-            return set_error(GL_E_UNSUPPORTED, "no device schedule for NTT size " + std::to_string(n));
-        it = g_plans_cols.emplace(n, std::move(p)).first;
-    }
-    *out = it->second.get();
-    return GL_OK;
-}
+int get_plan(u64 n, PlanRef *out) { return get_plan_impl(n, out, 0); }
+int get_plan_cols(u64 n, PlanRef *out) { return get_plan_impl(n, out, 1); }
 
 int set_plan(u64 n, const u32 *ids, size_t nids, bool cols) {
     if (n == 0 || (P - 1) % n != 0) return set_error(GL_E_SIZE, "plan_set: size does not divide p - 1");
+    std::call_once(g_tune_once, load_tune_file);
+    std::vector<u32> v(ids, ids + nids);
+    if (nids) {
+        Plan check;
+        if (!plan_from_ids(n, v, check)) return set_error(GL_E_ARG, "plan_set: unknown variant id or radices do not multiply to n");
+    }
     std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto &plans = cols ? g_plans_cols : g_plans;
-    if (nids == 0) {
-        plans.erase(n);
-        return GL_OK;
-    }
-    int cnt = 0;
-    const PassVariant *vs = pass_variants(&cnt);
-    std::unique_ptr<Plan> p(new Plan());
-    p->n = n;
-    u64 prod = 1;
-    for (size_t i = 0; i < nids; i++) {
-        const PassVariant *v = nullptr;
-        if (ids[i] < (u32)cnt) v = &vs[ids[i]];
-        else if (ids[i] == (u32)cnt) v = rader257_variant();
-        if (!v) return set_error(GL_E_ARG, "plan_set: unknown variant id");
-        u64 R = (u64)v->R1 * (u64)v->R2;
-        p->passes.push_back({v, R});
-        prod *= R;
-    }
-    if (prod != n) return set_error(GL_E_ARG, "plan_set: radices do not multiply to n");
-    plans[n] = std::move(p);
+    if (nids) g_explicit[cols][n] = v;
+    else g_explicit[cols].erase(n);
+    g_plans[cols].erase(n);  // holders keep their PlanRef
     return GL_OK;
 }
 
@@ -708,7 +755,12 @@ static void load_tune_file() {
             }
             ids.push_back((u32)id);
         }
-        if (ok && !ids.empty()) (void)set_plan(n, ids.data(), ids.size(), cols);
+        Plan check;
+        if (ok && !ids.empty() && n && (P - 1) % n == 0 && plan_from_ids(n, ids, check)) {
+            std::lock_guard<std::mutex> lk(g_plan_mu);
+            g_tuned[cols ? 1 : 0][n] = ids;
+            g_plans[cols ? 1 : 0].erase(n);
+        }
     }
     fclose(f);
 }
@@ -741,13 +793,16 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     fastdiv_params(S, a.S_mul, a.S_sh);
     fastdiv_params(S0, a.S0_mul, a.S0_sh);
     a.SS = S / S0;
+    TableRef inner;
     if (pp.v->kind == VK_RADER257) {
-        a.tw_inner = rader257_table();
+        inner = rader257_table();
     } else if (pp.v->kind == VK_THREE_STAGE) {
-        a.tw_inner = inner_table(R);
+        inner = inner_table(R);
     } else if (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1) {
-        a.tw_inner = inner_matrix((u64)pp.v->R1, (u64)pp.v->R2);
+        inner = inner_matrix((u64)pp.v->R1, (u64)pp.v->R2);
     }
+    if (inner) keep.push_back(inner);
+    a.tw_inner = inner.get();
     const bool needs_inner = pp.v->kind == VK_RADER257 || pp.v->kind == VK_THREE_STAGE ||
                              (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1);
     if (needs_inner && !a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
@@ -757,15 +812,18 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         // workgroup fetches one run of R words for T*R data words.
         const u64 fmax = full_table_max();
         const bool use_full = M <= fmax || (S >= 32 && M <= 4 * fmax);
-        const u64 *full = use_full ? twiddle_matrix(M, R) : nullptr;
+        TableRef full = use_full ? twiddle_matrix(M, R) : nullptr;
         if (full) {
-            a.tw_lo = full;
+            keep.push_back(full);
+            a.tw_lo = full.get();
             a.flags |= FLAG_TWIDDLE_FULL;
         } else {
             PowTable t = root_table(M);
             if (!t.lo || !t.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
-            a.tw_lo = t.lo;
-            a.tw_hi = t.hi;
+            keep.push_back(t.lo);
+            keep.push_back(t.hi);
+            a.tw_lo = t.lo.get();
+            a.tw_hi = t.hi.get();
             a.lo_bits = t.lo_bits;
             a.flags |= FLAG_TWIDDLE;
             // inverse: 1/n multiplies the first output twiddle of each thread's
@@ -930,38 +988,82 @@ int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstri
         if (rc) return rc;
         hipEvent_t e0 = timing_begin(stream);
         if (!v->launch) {
-            if ((rc = run_rader_big(a, stream))) return rc;
+            if ((rc = run_rader_big(a, stream, c.keep))) return rc;
         } else {
             v->launch(a, grid, stream);
             GL_HIP(hipGetLastError());
         }
         timing_end(v, e0, stream);
     }
-    return GL_OK;
+    return GL_OK;  // c.keep drops here: the passes are enqueued
+}
+
+int warm_plan(const Plan &plan, u64 S0, Keep *keep) {
+    int rc = GL_OK;
+    for (int inverse = 0; inverse < 2 && !rc; inverse++) {
+        // null data/tmp: next() only fills arguments, nothing is launched
+        PassCursor c{&plan, nullptr, nullptr, S0, 1, plan.n * S0, inverse != 0};
+        c.begin();
+        while (!c.done() && !rc) {
+            PassArgs a;
+            unsigned grid = 0;
+            rc = c.next(a, grid);
+        }
+        if (!rc) {
+            for (const PassPlan &pp : plan.passes) {
+                if (pp.v->launch) continue;
+                BigRaderTables t;
+                if ((rc = big_rader_tables(t))) break;
+                c.keep.push_back(t.permk);
+                c.keep.push_back(t.permi);
+                c.keep.push_back(t.spec);
+            }
+        }
+        if (keep) keep->insert(keep->end(), c.keep.begin(), c.keep.end());
+    }
+    return rc;
+}
+
+size_t plan_extra_words(const Plan &plan, u64 count, u64 bstride) {
+    // the radix-65537 pass takes C and its ping-pong copy (65536 words per
+    // instance each) and one word per instance
+    for (const PassPlan &pp : plan.passes)
+        if (!pp.v->launch) {
+            const u64 inst = count * bstride / 65537;
+            return (size_t)(2 * inst * 65536 + inst);
+        }
+    return 0;
 }
 
 void clear_caches() {
-    (void)hipDeviceSynchronize();
     {
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        g_plans.clear();
-        g_plans_cols.clear();
+        g_plans[0].clear();
+        g_plans[1].clear();
     }
     {
         std::lock_guard<std::mutex> lk(g_tw_mu);
-        for (void *p : g_tw_allocs) (void)hipFree(p);
-        g_tw_allocs.clear();
         g_pow_tables.clear();
         g_inner_tables.clear();
         g_inner_matrices.clear();
         g_coset_tables.clear();
         g_scaled_coset_tables.clear();
         g_full_tables.clear();
-        g_rader257 = nullptr;
+        g_rader257.reset();
         g_big_rader = BigRaderTables();
     }
-    release_scratch();
-    pool_trim();
+    // Tables retired up to now: their last user enqueued its work before
+    // the retirement, so once the device has drained that work is done.
+    // Tables still referenced (by a caller mid-enqueue or a captured batch
+    // plan) are retired later and freed by a later call.
+    std::vector<void *> dead;
+    {
+        std::lock_guard<std::mutex> lk(g_grave_mu);
+        dead.swap(g_grave);
+    }
+    if (hipDeviceSynchronize() != hipSuccess) (void)hipGetLastError();
+    for (void *p : dead) (void)hipFree(p);
+    pool_release_idle();
 }
 
 }  // namespace gl

@@ -1,9 +1,25 @@
 // Host runtime: error reporting, device memory pool, twiddle tables, the
 // NTT planner and plan execution.  Shared by the C ABI entry points.
+//
+// Lifetime rules (every entry point may run on several host threads):
+// - Plans are reference counted (PlanRef); replacing or clearing a cached
+//   plan never frees one a caller still holds.
+// - Device tables are reference counted (TableRef).  A table whose last
+//   reference drops is retired, not freed; retired tables are freed by
+//   clear_caches() after a device synchronization that follows their
+//   retirement, so work enqueued while a reference was held has finished.
+//   Callers hold the references of every table their enqueued work reads
+//   (PassCursor::keep) until the enqueue returns; captured graphs hold them
+//   for their lifetime.
+// - Device scratch comes from a stream-ordered pool: a block freed on
+//   stream s may be handed out again at once for work on s (stream order
+//   runs the earlier user first), and to another stream only after the
+//   event recorded at the free has completed.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -24,45 +40,57 @@ int hip_error(hipError_t e, const char *what);
 
 int ensure_init();
 
-// ---- device memory pool (buffers are reused by size class) ----
-void *pool_alloc(size_t bytes);
-void pool_free(void *p);
-void pool_trim();
+// ---- device memory pool (stream ordered, reused by size class) ----
+void *pool_alloc(size_t bytes, hipStream_t s);
+// Return a block; work enqueued on s before this call may still use it.
+void pool_free(void *p, hipStream_t s);
+// Free every idle block whose last use has completed (no device sync).
+void pool_release_idle();
 
+// A pool block owned for the duration of one enqueue on a stream.
 struct DevBuf {
     void *p = nullptr;
+    hipStream_t s = nullptr;
     DevBuf() = default;
-    explicit DevBuf(size_t bytes) : p(bytes ? pool_alloc(bytes) : nullptr) {}
+    DevBuf(size_t bytes, hipStream_t stream) : p(bytes ? pool_alloc(bytes, stream) : nullptr), s(stream) {}
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { if (p) pool_free(p); }
+    ~DevBuf() { if (p) pool_free(p, s); }
     template <class T> T *as() const { return (T *)p; }
 };
 
 hipStream_t thread_stream();
 
-// Scratch words owned by a stream: work enqueued on one stream runs in order,
-// so successive calls on that stream can share the buffer without waiting.
-// Growing it waits for the stream before releasing the old buffer.
-u64 *stream_scratch(hipStream_t s, size_t words, int slot = 0);
+// ---- twiddles (device tables, Montgomery form, cached) ----
+using TableRef = std::shared_ptr<const u64>;
+using Keep = std::vector<TableRef>;
 
-// ---- twiddles ----
 struct PowTable {  // w^e = lo[e & (2^lo_bits - 1)] * hi[e >> lo_bits], e < order
-    const u64 *lo = nullptr, *hi = nullptr;
+    TableRef lo, hi;
     u32 lo_bits = 0;
 };
 // Powers of w_M (root of order M).
 PowTable root_table(u64 M);
 // Powers of an arbitrary base g for exponents < count (e.g. coset shifts).
 PowTable power_table(u64 base, u64 count);
-// w_R^e for e < R (inner twiddles of a two-sub-step pass).
-const u64 *inner_table(u64 R);
+// w_R^e for e < R (inner twiddles of a three-stage pass).
+TableRef inner_table(u64 R);
+// Inner twiddles of a two-sub-step pass R1 x R2: entry rb*R1 + ka = w_R^(ka*rb).
+TableRef inner_matrix(u64 R1, u64 R2);
+// Output-order twiddles of a pass of radix R on sub-problems of length M:
+// entry k + R*m = w_M^(m*k), M words.
+TableRef twiddle_matrix(u64 M, u64 R);
 // Largest sub-problem length M whose output twiddles use a full M-word
 // table; returns the previous value.
 u64 set_full_table_max(u64 words);
 const PassVariant *rader65537_variant();
 // Tables of the radix-257 Rader pass (see gl_rader.hip).
-const u64 *rader257_table();
+TableRef rader257_table();
+// Montgomery 2717^j for j < rows (the commit's coset shift).
+TableRef coset_table(u64 rows);
+// Montgomery 2717^j / rows for j < rows: the intt scale and the coset shift
+// of the commit's extension in one factor per row.
+TableRef scaled_coset_table(u64 rows);
 
 // ---- planning ----
 struct PassPlan {
@@ -73,19 +101,22 @@ struct Plan {
     u64 n = 0;
     std::vector<PassPlan> passes;
 };
+using PlanRef = std::shared_ptr<const Plan>;
 // Returns GL_OK and a pass list for length n, GL_E_SIZE or GL_E_UNSUPPORTED.
-int get_plan(u64 n, const Plan **out);
+// The schedule is the explicit one installed by set_plan, else the tune
+// file's, else the planner's default.
+int get_plan(u64 n, PlanRef *out);
 // Schedule for column transforms (S0 > 1 interleaved transforms).
-int get_plan_cols(u64 n, const Plan **out);
+int get_plan_cols(u64 n, PlanRef *out);
 // Install an explicit pass list (variant indices as listed by gl_variants;
-// index == variant count means the radix-257 Rader pass); empty restores
-// the planner's default.
+// index == variant count means the radix-257 Rader pass); an empty list
+// removes it, so the tune file's schedule (else the default) applies again.
 // cols: the schedule of column transforms (S0 > 1) instead of contiguous ones
 int set_plan(u64 n, const u32 *ids, size_t nids, bool cols = false);
 
-// Pass p of a plan as launch arguments.  `src_is_data` tracks the ping-pong
-// state: pass 0 reads data; non-final passes alternate data <-> tmp; the
-// final pass writes data (in place when it also reads data).
+// Pass p of a plan as launch arguments.  Ping-pong: pass 0 reads data (or
+// `input`); non-final passes alternate data <-> tmp; the final pass writes
+// data (in place when it also reads data).
 struct PassCursor {
     const Plan *plan;
     u64 *data, *tmp;
@@ -106,6 +137,7 @@ struct PassCursor {
     u64 S = 0, M = 0;
     const u64 *src = nullptr;
     bool scale_folded = false;  // 1/n went into the first pass's twiddle recurrence
+    Keep keep;                  // tables the filled-in passes read
     void begin() { p = 0; S = S0; M = plan->n; src = input ? input : data; scale_folded = false; }
     bool done() const { return p >= plan->passes.size(); }
     // Fills args/grid for the current pass and advances; returns GL_OK or an error.
@@ -120,16 +152,18 @@ struct PassCursor {
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
              hipStream_t stream, bool scale_out = true, const u64 *input = nullptr, const u64 *coset = nullptr,
              u64 coset_rows = 0, const u64 *row_scale = nullptr);
-// Montgomery 2717^j for j < rows (device, cached).
-const u64 *coset_table(u64 rows);
-// Montgomery 2717^j / rows for j < rows (device, cached): the intt scale and
-// the coset shift of the commit's extension in one factor per row.
-const u64 *scaled_coset_table(u64 rows);
+// Build every table a plan's passes read (forward and inverse), without
+// launching anything; the references go to keep.
+int warm_plan(const Plan &plan, u64 S0, Keep *keep);
 // True when the plan contains a pass that is not a single kernel (the
 // radix-65537 Rader pass), which grouped batch launches cannot include.
 bool plan_is_composite(const Plan &plan);
 size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
+// Device words of scratch a plan takes beyond tmp (the radix-65537 pass).
+size_t plan_extra_words(const Plan &plan, u64 count, u64 bstride);
 
+// Drop cached plans and tables (explicit schedules stay installed), free
+// retired tables after a device synchronization, release idle pool blocks.
 void clear_caches();
 
 // Per-pass timing (gl_timing_enable / gl_timing_collect).
@@ -143,6 +177,8 @@ int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count);
 // Commit pieces shared by the C ABI and the server (gl_api.cpp).
 int commit_check(size_t rows, size_t cols, unsigned blowup);
 size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup);
+// Plans and tables of a commit shape (built once; references in keep).
+int commit_prepare(size_t rows, size_t cols, unsigned blowup, Keep *keep);
 int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
                    u64 *coef, u64 *tmp, hipStream_t s);
 

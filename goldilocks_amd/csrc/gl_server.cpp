@@ -6,8 +6,12 @@
 // enqueued whole on their stream (copies in, passes, copies out, an event),
 // so jobs that arrive while others run overlap with them on the device.
 // Completion is observed by polling the job's event; the job's buffers then
-// return to the device pool and its budget is released.  Plans and twiddle
-// tables are cached process-wide and stay resident across jobs.
+// return to the device pool and its budget is released.
+//
+// Plans and twiddle tables are built at submission, in the submitting
+// thread (the job keeps references to them), so the scheduler never builds
+// a table or waits for the device; it enqueues a job without holding the
+// server lock, so submit/poll/wait/stats are never stalled by an enqueue.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -15,12 +19,17 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
 #include "gl_runtime.hpp"
 
 using namespace gl;
+
+namespace gl {
+const char *last_error_cstr();
+}
 
 namespace {
 
@@ -35,7 +44,10 @@ struct Job {
     size_t footprint = 0;
     double t_submit = 0, t_done = 0;
     int status = GL_OK;
+    std::string msg;  // error message of a failed job
     bool done = false;
+    PlanRef plan;  // NTT jobs
+    Keep keep;     // tables the job's passes read
     // in flight
     int slot = -1;
     std::vector<void *> bufs;
@@ -50,6 +62,7 @@ struct Slot {
 size_t ntt_footprint(const gl_job &j, const Plan *plan) {
     size_t words = j.n * j.count;
     size_t f = plan && plan->passes.size() > 1 ? words * 8 : 0;
+    if (plan) f += plan_extra_words(*plan, j.count, j.n) * 8;
     if (!j.device) f += words * 8;
     return f;
 }
@@ -61,6 +74,9 @@ size_t commit_footprint(const gl_job &j) {
     return f;
 }
 
+// Latencies of the most recent completions (a bounded reservoir).
+constexpr size_t kLatencyWindow = 1 << 16;
+
 }  // namespace
 
 struct gl_server {
@@ -69,13 +85,14 @@ struct gl_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Job>> queue;
-    std::map<uint64_t, std::shared_ptr<Job>> jobs;
+    std::map<uint64_t, std::shared_ptr<Job>> jobs;  // until wait/poll reports them done
     uint64_t next_ticket = 1;
     bool stop = false;
     std::thread worker;
     // stats
     uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0;
-    std::vector<double> latencies;
+    std::vector<double> latencies;  // ring of kLatencyWindow entries
+    size_t lat_next = 0;
     double elements = 0, busy_ms = 0, busy_since = -1;
     int inflight = 0;
 
@@ -84,20 +101,20 @@ struct gl_server {
     void finish_job(Slot &slot, int status);
 };
 
+// Enqueue a job on its slot's stream (no server lock held).
 int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
     const gl_job &j = job->spec;
     hipStream_t s = slot.stream;
     auto alloc = [&](size_t bytes) -> void * {
-        void *p = bytes ? pool_alloc(bytes) : nullptr;
+        void *p = bytes ? pool_alloc(bytes, s) : nullptr;
         if (p) job->bufs.push_back(p);
         return p;
     };
     int rc = GL_OK;
     if (j.kind == GL_JOB_NTT || j.kind == GL_JOB_INTT) {
         const bool inverse = j.kind == GL_JOB_INTT;
-        if (inverse && j.n <= 1) return GL_OK;
-        const Plan *plan = nullptr;
-        if ((rc = get_plan(j.n, &plan))) return rc;
+        if (!job->plan) return GL_OK;  // intt of n <= 1
+        const Plan &plan = *job->plan;
         size_t words = j.n * j.count;
         u64 *data = j.data;
         if (!j.device) {
@@ -106,11 +123,11 @@ int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
             GL_HIP(hipMemcpyAsync(data, j.data, words * 8, hipMemcpyHostToDevice, s));
         }
         u64 *tmp = nullptr;
-        if (plan->passes.size() > 1) {
+        if (plan.passes.size() > 1) {
             tmp = (u64 *)alloc(words * 8);
             if (!tmp) return set_error(GL_E_NOMEM, "server: allocation failed");
         }
-        if ((rc = run_plan(*plan, data, tmp, 1, j.count, j.n, inverse, s))) return rc;
+        if ((rc = run_plan(plan, data, tmp, 1, j.count, j.n, inverse, s))) return rc;
         if (!j.device) GL_HIP(hipMemcpyAsync(j.data, data, words * 8, hipMemcpyDeviceToHost, s));
     } else {
         size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
@@ -137,15 +154,19 @@ int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
     return GL_OK;
 }
 
+// Caller holds mu.
 void gl_server::finish_job(Slot &slot, int status) {
     std::shared_ptr<Job> job = slot.job;
     slot.job.reset();
-    for (void *p : job->bufs) pool_free(p);
+    for (void *p : job->bufs) pool_free(p, slot.stream);
     job->bufs.clear();
+    job->keep.clear();
+    job->plan.reset();
     if (job->event) (void)hipEventDestroy(job->event);
     job->event = nullptr;
     job->t_done = now_ms();
     job->status = status;
+    if (status != GL_OK) job->msg = last_error_cstr();
     job->done = true;
     in_use -= job->footprint;
     inflight--;
@@ -155,7 +176,10 @@ void gl_server::finish_job(Slot &slot, int status) {
     }
     if (status == GL_OK) {
         completed++;
-        latencies.push_back(job->t_done - job->t_submit);
+        const double lat = job->t_done - job->t_submit;
+        if (latencies.size() < kLatencyWindow) latencies.push_back(lat);
+        else latencies[lat_next] = lat;
+        lat_next = (lat_next + 1) % kLatencyWindow;
         const gl_job &j = job->spec;
         elements += j.kind == GL_JOB_COMMIT ? (double)j.n * j.blowup * j.count : (double)j.n * j.count;
     } else {
@@ -169,7 +193,7 @@ void gl_server::loop() {
         // completions
         bool progressed = false;
         for (Slot &sl : slots) {
-            if (!sl.job) continue;
+            if (!sl.job || !sl.job->event) continue;  // idle, or being enqueued
             hipError_t q = hipEventQuery(sl.job->event);
             if (q == hipErrorNotReady) continue;
             finish_job(sl, q == hipSuccess ? GL_OK : hip_error(q, "server job"));
@@ -192,13 +216,18 @@ void gl_server::loop() {
             in_use += job->footprint;
             peak = std::max(peak, in_use);
             if (inflight++ == 0) busy_since = now_ms();
+            // enqueue without the lock; the slot is ours (job set, no event yet)
+            lk.unlock();
             int rc = start_job(*free_slot, job);
+            hipEvent_t ev = nullptr;
             hipError_t e = hipSuccess;
-            if (rc == GL_OK) e = hipEventCreateWithFlags(&job->event, hipEventDisableTiming);
-            if (rc == GL_OK && e == hipSuccess) e = hipEventRecord(job->event, free_slot->stream);
+            if (rc == GL_OK) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (rc == GL_OK && e == hipSuccess) e = hipEventRecord(ev, free_slot->stream);
             if (rc == GL_OK && e != hipSuccess) rc = hip_error(e, "server event");
+            if (rc != GL_OK) (void)hipStreamSynchronize(free_slot->stream);
+            lk.lock();
+            job->event = ev;
             if (rc != GL_OK) {
-                (void)hipStreamSynchronize(free_slot->stream);
                 finish_job(*free_slot, rc);
                 cv_done.notify_all();
             }
@@ -239,18 +268,24 @@ int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket) {
     auto j = std::make_shared<Job>();
     j->spec = *job;
     const gl_job &s = j->spec;
+    // plans and tables are built here, in the submitting thread
     if (s.kind == GL_JOB_NTT || s.kind == GL_JOB_INTT) {
-        const Plan *plan = nullptr;
         if (!(s.kind == GL_JOB_INTT && s.n <= 1)) {
-            int rc = get_plan(s.n, &plan);
+            int rc = get_plan(s.n, &j->plan);
             if (rc) return rc;
+            if (j->plan->passes.empty()) j->plan.reset();
         }
         if (s.count == 0 || (!s.data && s.n)) return set_error(GL_E_ARG, "server: empty or null NTT job");
-        j->footprint = ntt_footprint(s, plan);
+        if (j->plan) {
+            int rc = warm_plan(*j->plan, 1, &j->keep);
+            if (rc) return rc;
+        }
+        j->footprint = ntt_footprint(s, j->plan.get());
     } else if (s.kind == GL_JOB_COMMIT) {
         int rc = commit_check(s.n, s.count, s.blowup);
         if (rc) return rc;
         if (!s.data || !s.lde_out || !s.layers_out) return set_error(GL_E_ARG, "server: null commit buffer");
+        if ((rc = commit_prepare(s.n, s.count, s.blowup, &j->keep))) return rc;
         j->footprint = commit_footprint(s);
     } else {
         return set_error(GL_E_ARG, "server: unknown job kind");
@@ -275,8 +310,13 @@ int gl_server_poll(gl_server *srv, uint64_t ticket, int *done) {
     std::lock_guard<std::mutex> lk(srv->mu);
     auto it = srv->jobs.find(ticket);
     if (it == srv->jobs.end()) return set_error(GL_E_ARG, "unknown ticket");
-    if (done) *done = it->second->done;
-    return it->second->done ? it->second->status : GL_OK;
+    const bool fin = it->second->done;
+    const int status = fin ? it->second->status : GL_OK;
+    const std::string it_msg = it->second->msg;
+    if (done) *done = fin;
+    if (fin) srv->jobs.erase(it);  // reported: the ticket is released
+    if (status != GL_OK) return set_error(status, "server job: " + it_msg);
+    return status;
 }
 
 int gl_server_wait(gl_server *srv, uint64_t ticket, double *latency_ms) {
@@ -287,7 +327,9 @@ int gl_server_wait(gl_server *srv, uint64_t ticket, double *latency_ms) {
     std::shared_ptr<Job> j = it->second;
     srv->cv_done.wait(lk, [&] { return j->done; });
     if (latency_ms) *latency_ms = j->t_done - j->t_submit;
-    return j->status;
+    srv->jobs.erase(ticket);  // reported: the ticket is released
+    if (j->status != GL_OK) return set_error(j->status, "server job: " + j->msg);
+    return GL_OK;
 }
 
 int gl_server_drain(gl_server *srv) {
@@ -299,14 +341,20 @@ int gl_server_drain(gl_server *srv) {
 
 int gl_server_stats_get(gl_server *srv, gl_server_stats *out) {
     if (!srv || !out) return set_error(GL_E_ARG, "null pointer");
-    std::lock_guard<std::mutex> lk(srv->mu);
     gl_server_stats st{};
-    st.submitted = srv->submitted;
-    st.completed = srv->completed;
-    st.failed = srv->failed;
-    st.rejected = srv->rejected;
-    std::vector<double> l = srv->latencies;
-    std::sort(l.begin(), l.end());
+    std::vector<double> l;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        st.submitted = srv->submitted;
+        st.completed = srv->completed;
+        st.failed = srv->failed;
+        st.rejected = srv->rejected;
+        st.elements = srv->elements;
+        st.busy_seconds = srv->busy_ms / 1e3;
+        st.peak_bytes = srv->peak;
+        l = srv->latencies;  // at most kLatencyWindow entries
+    }
+    std::sort(l.begin(), l.end());  // outside the lock
     if (!l.empty()) {
         auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
         st.p50_ms = pct(0.50);
@@ -316,9 +364,6 @@ int gl_server_stats_get(gl_server *srv, gl_server_stats *out) {
         st.mean_ms = sum / l.size();
         st.max_ms = l.back();
     }
-    st.elements = srv->elements;
-    st.busy_seconds = srv->busy_ms / 1e3;
-    st.peak_bytes = srv->peak;
     *out = st;
     return GL_OK;
 }
