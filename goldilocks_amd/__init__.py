@@ -69,6 +69,7 @@ def lib():
             "gl_ntt_batch": (i32, [vp, sz, sz, i32]),
             "gl_ntt_batch_mixed": (i32, [vp, vp, sz, i32]),
             "gl_ntt_dev": (i32, [vp, sz, sz, i32, vp]),
+            "gl_ntt_dev_noalloc": (i32, [vp, sz, sz, i32, vp]),
             "gl_ntt_cols_dev": (i32, [vp, sz, sz, sz, i32, vp]),
             "gl_ntt_mixed_dev": (i32, [vp, vp, sz, i32, vp]),
             "gl_ntt_mixed_dev_unbatched": (i32, [vp, vp, sz, i32, vp]),

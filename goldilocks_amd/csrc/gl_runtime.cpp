@@ -273,11 +273,12 @@ static TableRef generate(u64 words, F gen) {
     return adopt(d);
 }
 
-PowTable power_table(u64 base, u64 count) {
+PowTable power_table(u64 base, u64 count, bool create) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto key = std::make_pair(base, count);
     auto it = g_pow_tables.find(key);
     if (it != g_pow_tables.end()) return it->second;
+    if (!create) return PowTable();
     u32 lo_bits = (u32)((bit_length(count > 1 ? count - 1 : 1) + 1) / 2);
     u64 L = 1ull << lo_bits;
     u64 H = (count + L - 1) / L;
@@ -300,9 +301,14 @@ PowTable power_table(u64 base, u64 count) {
     return t;
 }
 
-PowTable root_table(u64 M) { return power_table(root(M), M); }
+PowTable root_table(u64 M, bool create) { return power_table(root(M), M, create); }
 
-TableRef twiddle_matrix(u64 M, u64 R) {
+TableRef twiddle_matrix(u64 M, u64 R, bool create) {
+    if (!create) {
+        std::lock_guard<std::mutex> lk(g_tw_mu);
+        auto it = g_full_tables.find(std::make_pair(M, R));
+        return it != g_full_tables.end() ? it->second : nullptr;
+    }
     PowTable t = root_table(M);
     if (!t.lo || !t.hi) return nullptr;
     std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -331,10 +337,11 @@ static std::atomic<u64> &full_table_max_ref() {
 static u64 full_table_max() { return full_table_max_ref().load(); }
 u64 set_full_table_max(u64 words) { return full_table_max_ref().exchange(words); }
 
-TableRef inner_table(u64 R) {
+TableRef inner_table(u64 R, bool create) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_inner_tables.find(R);
     if (it != g_inner_tables.end()) return it->second;
+    if (!create) return nullptr;
     std::vector<u64> t(R);
     u64 w = root(R), v = 1;
     for (u64 e = 0; e < R; e++) {
@@ -372,11 +379,12 @@ TableRef scaled_coset_table(u64 rows) {
     return d;
 }
 
-TableRef inner_matrix(u64 R1, u64 R2) {
+TableRef inner_matrix(u64 R1, u64 R2, bool create) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto key = std::make_pair(R1, R2);
     auto it = g_inner_matrices.find(key);
     if (it != g_inner_matrices.end()) return it->second;
+    if (!create) return nullptr;
     const u64 R = R1 * R2;
     std::vector<u64> pw(R), t(R);
     u64 w = root(R), v = 1;
@@ -395,9 +403,9 @@ TableRef inner_matrix(u64 R1, u64 R2) {
 // [0, 256) w_256^(k1*j) at entry j*16 + k1 (a thread's inner twiddles are
 // consecutive), [256, 512) the convolution spectrum DFT_256(w_257^(3^q)/256),
 // [512, 768) the conjugate inner twiddles of the second (inverse) DFT_256.
-TableRef rader257_table() {
+TableRef rader257_table(bool create) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
-    if (g_rader257) return g_rader257;
+    if (g_rader257 || !create) return g_rader257;
     std::vector<u64> pw(256), t(768);
     u64 w256 = root(256), w257 = root(257), s256 = inv(256);
     u64 v = 1;
@@ -793,33 +801,41 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
     fastdiv_params(S, a.S_mul, a.S_sh);
     fastdiv_params(S0, a.S0_mul, a.S0_sh);
     a.SS = S / S0;
+    const bool mk = create_tables;
+    static const char *kNotResident =
+        "twiddle tables for this length are not resident and this call allocates nothing: "
+        "run the allocating entry point on the same length once first";
     TableRef inner;
     if (pp.v->kind == VK_RADER257) {
-        inner = rader257_table();
+        inner = rader257_table(mk);
     } else if (pp.v->kind == VK_THREE_STAGE) {
-        inner = inner_table(R);
+        inner = inner_table(R, mk);
     } else if (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1) {
-        inner = inner_matrix((u64)pp.v->R1, (u64)pp.v->R2);
+        inner = inner_matrix((u64)pp.v->R1, (u64)pp.v->R2, mk);
     }
     if (inner) keep.push_back(inner);
     a.tw_inner = inner.get();
     const bool needs_inner = pp.v->kind == VK_RADER257 || pp.v->kind == VK_THREE_STAGE ||
                              (pp.v->kind == VK_TWO_STAGE && pp.v->R2 > 1);
-    if (needs_inner && !a.tw_inner) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+    if (needs_inner && !a.tw_inner)
+        return mk ? set_error(GL_E_NOMEM, "twiddle table allocation failed") : set_error(GL_E_UNSUPPORTED, kNotResident);
     if (Mp > 1) {
         // Past the cache-resident size a full matrix still pays off when S >= 32:
         // the S adjacent instances s + S*m share each of its words, so every
         // workgroup fetches one run of R words for T*R data words.
         const u64 fmax = full_table_max();
         const bool use_full = M <= fmax || (S >= 32 && M <= 4 * fmax);
-        TableRef full = use_full ? twiddle_matrix(M, R) : nullptr;
+        TableRef full = use_full ? twiddle_matrix(M, R, mk) : nullptr;
+        if (use_full && !full && !mk) return set_error(GL_E_UNSUPPORTED, kNotResident);
         if (full) {
             keep.push_back(full);
             a.tw_lo = full.get();
             a.flags |= FLAG_TWIDDLE_FULL;
         } else {
-            PowTable t = root_table(M);
-            if (!t.lo || !t.hi) return set_error(GL_E_NOMEM, "twiddle table allocation failed");
+            PowTable t = root_table(M, mk);
+            if (!t.lo || !t.hi)
+                return mk ? set_error(GL_E_NOMEM, "twiddle table allocation failed")
+                          : set_error(GL_E_UNSUPPORTED, kNotResident);
             keep.push_back(t.lo);
             keep.push_back(t.hi);
             a.tw_lo = t.lo.get();

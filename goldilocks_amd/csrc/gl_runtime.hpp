@@ -69,23 +69,24 @@ struct PowTable {  // w^e = lo[e & (2^lo_bits - 1)] * hi[e >> lo_bits], e < orde
     TableRef lo, hi;
     u32 lo_bits = 0;
 };
+// create = false: return the cached table or nullptr, never build one.
 // Powers of w_M (root of order M).
-PowTable root_table(u64 M);
+PowTable root_table(u64 M, bool create = true);
 // Powers of an arbitrary base g for exponents < count (e.g. coset shifts).
-PowTable power_table(u64 base, u64 count);
+PowTable power_table(u64 base, u64 count, bool create = true);
 // w_R^e for e < R (inner twiddles of a three-stage pass).
-TableRef inner_table(u64 R);
+TableRef inner_table(u64 R, bool create = true);
 // Inner twiddles of a two-sub-step pass R1 x R2: entry rb*R1 + ka = w_R^(ka*rb).
-TableRef inner_matrix(u64 R1, u64 R2);
+TableRef inner_matrix(u64 R1, u64 R2, bool create = true);
 // Output-order twiddles of a pass of radix R on sub-problems of length M:
 // entry k + R*m = w_M^(m*k), M words.
-TableRef twiddle_matrix(u64 M, u64 R);
+TableRef twiddle_matrix(u64 M, u64 R, bool create = true);
 // Largest sub-problem length M whose output twiddles use a full M-word
 // table; returns the previous value.
 u64 set_full_table_max(u64 words);
 const PassVariant *rader65537_variant();
 // Tables of the radix-257 Rader pass (see gl_rader.hip).
-TableRef rader257_table();
+TableRef rader257_table(bool create = true);
 // Montgomery 2717^j for j < rows (the commit's coset shift).
 TableRef coset_table(u64 rows);
 // Montgomery 2717^j / rows for j < rows: the intt scale and the coset shift
@@ -123,6 +124,7 @@ struct PassCursor {
     u64 S0, count, bstride;
     bool inverse;
     bool scale_out = true;  // inverse: multiply the last pass by 1/n
+    bool create_tables = true;  // false: a missing table is GL_E_UNSUPPORTED (nothing is allocated)
     const u64 *input = nullptr;  // first pass reads here instead of data (out of place)
     // first pass only (two-sub-step variants): input row j < coset_rows is
     // multiplied by coset[j], rows above read as zero (FLAG_COSET)

@@ -77,6 +77,18 @@ int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k, int invers
 /* As gl_ntt_batch on a device buffer of n*count words, enqueued on `stream`. */
 int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream);
 
+/* gl_ntt_dev that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It has no counterpart in the reference.  The ping-pong
+ * scratch is one fixed array of 2^24 words (128 MiB) per device, held until
+ * the process ends: n * count above 2^24 is GL_E_ARG, lengths with a factor
+ * 65537 are GL_E_UNSUPPORTED.  It builds no twiddle table: run an allocating
+ * entry point (gl_ntt_dev, gl_ntt, ...) on the same length once first, else
+ * GL_E_UNSUPPORTED.  Every call on a device shares its array: calls on one
+ * stream run in order, calls that overlap on one device (other streams or
+ * threads) race. */
+int gl_ntt_dev_noalloc(uint64_t *d, size_t n, size_t count, int inverse, void *stream);
+
 /* Column transforms of a row-major (n x cols) matrix: column c is words
  * c, c + cols, c + 2*cols, ...; `count` such matrices back to back. */
 int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int inverse, void *stream);
