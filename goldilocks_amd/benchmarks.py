@@ -56,6 +56,22 @@ def batch_summary(torch, gl, batch_k=(10, 20), count=1024, seed=7):
     plan.close()
     res["graph"] = {"p50_ms": round(_pct(times, 0.5), 3), "p99_ms": round(_pct(times, 0.99), 3),
                     "reps": len(times), "gelem_s": round(total / (_pct(times, 0.5) * 1e-3) / 1e9, 3)}
+    # Summed kernel time of one more repetition of each eager mode (every pass
+    # launch bracketed by HIP events): the floor no launch scheme can beat
+    # with these kernels.  wall / kernel is the most any batching of the
+    # same passes could gain over that mode.
+    for key, fn in (("batched", lambda: gl.ntt_batch_mixed(bufs)),
+                    ("unbatched", lambda: gl.ntt_batch_mixed(bufs, batched=False))):
+        gl.timing_collect()
+        gl.timing_enable(True)
+        fn()
+        torch.cuda.synchronize()
+        gl.timing_enable(False)
+        recs = gl.timing_collect()
+        kern = sum(ms for _, ms in recs)
+        res[key]["kernel_ms"] = round(kern, 3)
+        res[key]["launches"] = len(recs)
+        res[key]["wall_over_kernel"] = round(res[key]["p50_ms"] / kern, 3) if kern else None
     del bufs
     return {
         "workload": f"{count} forward NTTs, n = 2^k * f, k uniform in [{lo},{hi}], f in {{1,3,5,15,17,257}}, "
@@ -207,6 +223,18 @@ def bench_serve(args, torch, gl):
         st = srv.stats()
     lat.sort()
     total = sum(elems)
+    # The plain path for the same jobs: one job at a time through the
+    # ordinary entry points on one stream, waiting for each (what a client
+    # without the server does).
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for (kind, n, c), b in zip(jobs, bufs):
+        if kind == "ntt":
+            gl.strategy(n).ntt(b[0])
+        else:
+            gl.commit(b[0], 2)
+        torch.cuda.synchronize()
+    serial_s = time.perf_counter() - t1
     return {
         "metric": "Gelem/s and p50/p99 latency, served mixed NTT/commit jobs (open-loop Poisson)",
         "value": round(total / wall / 1e9, 3),
@@ -222,6 +250,9 @@ def bench_serve(args, torch, gl):
         "latency_ms": {"p50": round(lat[len(lat) // 2], 4), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 4),
                        "max": round(lat[-1], 4)},
         "wall_s": round(wall, 4),
+        "serial": {"wall_s": round(serial_s, 4), "gelem_s": round(total / serial_s / 1e9, 3),
+                   "path": "the same jobs one at a time through gl.ntt / gl.commit, synchronized per job"},
+        "speedup_vs_serial": round(serial_s / wall, 3),
         "elements": total,
         "admitted": int(st["completed"] - base["completed"]),
         "rejected": int(st["rejected"] - base["rejected"]),
