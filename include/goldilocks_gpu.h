@@ -12,11 +12,14 @@
  * reference panics (ntt/src/ntt/mod.rs:55-58, pcs/src/k12/generic.rs:65-66),
  * these return GL_E_SIZE / GL_E_ARG.
  *
- * Threading: all entry points may be called from several host threads; the
- * plan and twiddle caches are mutex guarded and host-buffer calls use a
- * stream per calling thread.  Device-buffer calls ("_dev") enqueue on the
- * caller's stream (NULL = the default stream) and return without
- * synchronizing unless stated otherwise.
+ * Threading: all entry points may be called from several host threads,
+ * also on one stream; host-buffer calls use a stream per calling thread.
+ * Plans and twiddle tables are shared and reference counted: a call, or a
+ * captured batch plan, keeps what it reads alive, and gl_clear_cache frees
+ * nothing still in use.  Each call takes its own device scratch, stream
+ * ordered (reused by later work on the same stream without waiting).
+ * Device-buffer calls ("_dev") enqueue on the caller's stream (NULL = the
+ * default stream) and return without synchronizing unless stated otherwise.
  */
 #ifndef GOLDILOCKS_GPU_H
 #define GOLDILOCKS_GPU_H
@@ -165,7 +168,10 @@ void gl_timing_enable(int on);
 size_t gl_timing_pending(void);
 int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *count);
 
-/* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46). */
+/* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46).
+ * Tables still referenced by work being enqueued or by a batch plan stay
+ * valid and are freed by a later call; explicit (gl_plan_set) and tuned
+ * schedules stay in force.  Synchronizes the device. */
 void gl_clear_cache(void);
 
 /* ---- K12 / Merkle (pcs/src/k12) ----------------------------------------- */
@@ -284,9 +290,11 @@ gl_server *gl_server_create(size_t hbm_budget_bytes, int slots);
 /* Queue a job; GL_E_NOMEM if its footprint alone exceeds the budget, GL_E_SIZE /
  * GL_E_ARG for invalid jobs.  The job's buffers must stay valid until done. */
 int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket);
-/* Nonblocking: *done = 1 when the job finished; returns the job's status. */
+/* Nonblocking: *done = 1 when the job finished; returns the job's status.
+ * Once a poll has reported the job done, its ticket is released. */
 int gl_server_poll(gl_server *srv, uint64_t ticket, int *done);
-/* Block until the job finished; latency_ms (may be NULL) = submit -> done. */
+/* Block until the job finished; latency_ms (may be NULL) = submit -> done.
+ * Returns the job's status and releases the ticket. */
 int gl_server_wait(gl_server *srv, uint64_t ticket, double *latency_ms);
 /* Block until every submitted job finished. */
 int gl_server_drain(gl_server *srv);
