@@ -240,13 +240,17 @@ static TableRef g_rader257;
 
 static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
+// Copy a host table to a new device allocation on the table stream (a
+// plain hipMemcpy would queue behind the caller's work on the NULL stream).
 static TableRef upload(const std::vector<u64> &h) {
+    hipStream_t ts = table_stream();
     void *d = nullptr;
-    if (hipMalloc(&d, h.size() * 8) != hipSuccess) {
+    if (!ts || hipMalloc(&d, h.size() * 8) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
-    if (hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, ts) != hipSuccess ||
+        hipStreamSynchronize(ts) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(d);
         return nullptr;
