@@ -14,8 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgoldilocks_amd.so")
-SOURCES = ["gl_ntt_kernels.hip", "gl_rader.hip", "gl_k12.hip", "gl_batch.hip", "gl_runtime.cpp", "gl_api.cpp",
-           "gl_server.cpp", "gl_noalloc.hip"]
+SOURCES = ["gl_ntt_kernels.hip"] + [f"gl_variants_{k}.hip" for k in range(8)] + [
+    "gl_rader.hip", "gl_k12.hip", "gl_batch.hip", "gl_runtime.cpp", "gl_api.cpp", "gl_server.cpp", "gl_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -42,7 +42,7 @@ def _compile(src):
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    jobs = jobs or min(8, os.cpu_count() or 4)
+    jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
