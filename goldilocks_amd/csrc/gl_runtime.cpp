@@ -88,9 +88,13 @@ size_t round_size(size_t b) {
     return (b + 4095) / 4096 * 4096;
 }
 
+// Work enqueued on s runs after earlier work on s.  hipStreamPerThread is
+// one handle for a different stream in every thread, so it never counts.
+bool same_stream(hipStream_t a, hipStream_t b) { return a == b && a != hipStreamPerThread; }
+
 // caller holds g_pool_mu
 bool block_ready(const FreeBlock &fb, hipStream_t s) {
-    if (!fb.ev || fb.stream == s) return true;
+    if (!fb.ev || same_stream(fb.stream, s)) return true;
     hipError_t q = hipEventQuery(fb.ev);
     if (q == hipSuccess) return true;
     if (q != hipErrorNotReady) (void)hipGetLastError();
@@ -110,7 +114,8 @@ void *pool_alloc(size_t bytes, hipStream_t s) {
         auto lo = g_pool_free.lower_bound(sz), hi = g_pool_free.upper_bound(sz + sz / 4);
         for (int pass = 0; pass < 2; pass++) {
             for (auto it = lo; it != hi; ++it) {
-                const bool ok = pass == 0 ? (!it->second.ev || it->second.stream == s) : block_ready(it->second, s);
+                const bool ok = pass == 0 ? (!it->second.ev || same_stream(it->second.stream, s))
+                                          : block_ready(it->second, s);
                 if (!ok) continue;
                 void *p = it->second.p;
                 recycle_event(it->second.ev);
