@@ -8,6 +8,8 @@ include/goldilocks_gpu.h (libgoldilocks_amd.so, loaded with ctypes):
   pcs crate (pcs/src/k12/mod.rs)    merkle_tree(data), process_layer(input),
                                     k12(data, H), next_layer_size(n)
   composed                          commit(matrix, blowup), ntt_batch_mixed(list)
+  binary tree of row hashes         row_tree(matrix), commit_rows(matrix, blowup),
+  (no reference function)           row_open(...), row_verify(...)
 
 `values` is a numpy uint64 array (host: copied to HBM and back) or a torch
 tensor on the GPU (int64/uint64 storage, transformed in place on the current
@@ -25,6 +27,7 @@ __all__ = [
     "P", "ntt", "intt", "strategy", "Strategy", "clear_cache", "is_divisor", "next_size",
     "merkle_tree", "process_layer", "k12", "next_layer_size", "merkle_tree_size",
     "commit", "ntt_batch_mixed", "fill_random", "lib", "GoldilocksError",
+    "commit_rows", "row_tree", "row_tree_size", "row_tree_depth", "row_layers", "row_open", "row_verify",
 ]
 
 P = 0xFFFFFFFF00000001
@@ -112,6 +115,14 @@ def lib():
             "gl_merkle_verify_dev": (i32, [vp, sz, vp, sz, vp, vp, vp]),
             "gl_commit": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz]),
             "gl_commit_dev": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz, vp]),
+            "gl_commit_rows": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz]),
+            "gl_commit_rows_dev": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, psz, vp]),
+            "gl_row_tree_size": (sz, [sz]),
+            "gl_row_tree_depth": (sz, [sz]),
+            "gl_row_tree": (i32, [vp, sz, sz, vp, sz, psz]),
+            "gl_row_tree_dev": (i32, [vp, sz, sz, vp, sz, psz, vp]),
+            "gl_row_open_dev": (i32, [vp, sz, vp, sz, vp, vp]),
+            "gl_row_verify_dev": (i32, [vp, sz, sz, vp, sz, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -417,6 +428,102 @@ def commit(matrix, blowup=2):
     return lde, _split_layers(out.raw[:need], ext * cols * 8)
 
 
+# ---------------------------------------------------------------- row tree
+def row_tree_size(leaves):
+    """Bytes of every layer of the row tree over `leaves` row hashes."""
+    return lib().gl_row_tree_size(leaves)
+
+
+def row_tree_depth(leaves):
+    """Layers above the leaves."""
+    return lib().gl_row_tree_depth(leaves)
+
+
+def row_layers(raw, leaves):
+    """Split the concatenated layers (leaves first) into a list, root last."""
+    layers, n, off = [], leaves, 0
+    while True:
+        layers.append(raw[off:off + 32 * n])
+        off += 32 * n
+        if n <= 1:
+            return layers
+        n = (n + 1) // 2
+
+
+def row_tree(matrix):
+    """Binary tree of row hashes of a row-major matrix of words (leaf r =
+    K12 of row r's bytes, parent = K12(left || right), the last node of an
+    odd layer moved up; gl_row_tree).  numpy input -> list of layer bytes,
+    leaves first; GPU tensor input -> one uint8 GPU tensor of all layers."""
+    nrows, cols = matrix.shape
+    need = row_tree_size(nrows)
+    got = ctypes.c_size_t(0)
+    if _is_torch(matrix):
+        import torch
+        out = torch.empty(max(need, 8) // 8, dtype=torch.int64, device=matrix.device).view(torch.uint8)
+        _check(lib().gl_row_tree_dev(_dev_ptr(matrix), nrows, cols, ctypes.c_void_p(out.data_ptr()), need,
+                                     ctypes.byref(got), _stream_ptr(matrix)))
+        return out[:need]
+    m = _host_u64(np.ascontiguousarray(matrix, dtype=np.uint64))
+    out = ctypes.create_string_buffer(max(need, 1))
+    _check(lib().gl_row_tree(m.ctypes.data_as(ctypes.c_void_p), nrows, cols, out, need, ctypes.byref(got)))
+    return row_layers(out.raw[:need], nrows)
+
+
+def commit_rows(matrix, blowup=2):
+    """commit's column coset extension with the row tree over its rows.
+
+    numpy input -> (lde ndarray, [layer bytes], leaves first); GPU tensor
+    input -> (lde tensor, layers uint8 tensor)."""
+    rows, cols = matrix.shape
+    ext = rows * blowup
+    need = row_tree_size(ext)
+    got = ctypes.c_size_t(0)
+    if _is_torch(matrix):
+        import torch
+        lde = torch.empty((ext, cols), dtype=matrix.dtype, device=matrix.device)
+        layers = torch.empty(max(need, 8) // 8, dtype=torch.int64, device=matrix.device).view(torch.uint8)
+        _check(lib().gl_commit_rows_dev(_dev_ptr(matrix), rows, cols, blowup, _dev_ptr(lde),
+                                        ctypes.c_void_p(layers.data_ptr()), need, ctypes.byref(got),
+                                        _stream_ptr(matrix)))
+        return lde, layers[:need]
+    m = _host_u64(np.ascontiguousarray(matrix, dtype=np.uint64))
+    lde = np.zeros((ext, cols), dtype=np.uint64)
+    out = ctypes.create_string_buffer(max(need, 1))
+    _check(lib().gl_commit_rows(m.ctypes.data_as(ctypes.c_void_p), rows, cols, blowup,
+                                lde.ctypes.data_as(ctypes.c_void_p), out, need, ctypes.byref(got)))
+    return lde, row_layers(out.raw[:need], ext)
+
+
+def row_open(layers, leaves, idx):
+    """Openings of leaves `idx` (int64 GPU tensor) from the row tree's layers
+    (the uint8 GPU tensor row_tree / commit_rows returned): a uint8 GPU
+    tensor (len(idx), depth, 32) of sibling hashes, zeros where a node moves
+    up unchanged."""
+    import torch
+    depth = row_tree_depth(leaves)
+    k = idx.numel()
+    out = torch.zeros((k, depth * 4), dtype=torch.int64, device=layers.device)
+    _check(lib().gl_row_open_dev(_bytes_dev_ptr(layers), leaves, _dev_ptr(idx), k, _dev_ptr(out),
+                                 _stream_ptr(layers)))
+    return out.view(torch.uint8).view(k, depth, 32)
+
+
+def row_verify(rows, leaves, idx, paths, root):
+    """Check opened rows (k x cols GPU tensor of the rows' words) at leaves
+    `idx` against openings (layout of row_open) and a 32-byte root; returns a
+    bool GPU tensor, one flag per opening."""
+    import torch
+    k, cols = rows.shape
+    ok = torch.zeros(k, dtype=torch.int32, device=rows.device)
+    root8 = torch.empty(4, dtype=torch.int64, device=rows.device)
+    root8.view(torch.uint8).copy_(root.reshape(-1)[:32])
+    paths8 = paths.contiguous()
+    _check(lib().gl_row_verify_dev(_dev_ptr(rows), cols, leaves, _dev_ptr(idx), k, _bytes_dev_ptr(paths8),
+                                   _dev_ptr(root8), _bytes_dev_ptr(ok), _stream_ptr(rows)))
+    return ok.bool()
+
+
 # ---------------------------------------------------------------- PoW / convolution
 
 def threshold_from_bits(bits):
@@ -518,7 +625,7 @@ def merkle_verify(paths, leaf_len, blocks, root):
 
 # ---------------------------------------------------------------- serving
 
-GL_JOB_NTT, GL_JOB_INTT, GL_JOB_COMMIT = 0, 1, 2
+GL_JOB_NTT, GL_JOB_INTT, GL_JOB_COMMIT, GL_JOB_COMMIT_ROWS = 0, 1, 2, 3
 
 
 class _Job(ctypes.Structure):
@@ -565,16 +672,18 @@ class Server:
         job = _Job(GL_JOB_INTT if inverse else GL_JOB_NTT, int(dev), ptr, n, _len(values) // n, 0, None, None)
         return self._submit(job, values)
 
-    def submit_commit(self, matrix, blowup, lde_out, layers_out):
+    def submit_commit(self, matrix, blowup, lde_out, layers_out, row_tree=False):
         """Commit a rows x cols matrix; lde_out ((blowup*rows) x cols) and
-        layers_out (merkle_tree_size bytes) receive the results."""
+        layers_out (merkle_tree_size bytes, or row_tree_size(blowup*rows)
+        with row_tree=True: the commit_rows form) receive the results."""
         rows, cols = matrix.shape
         dev = _is_torch(matrix)
         if dev:
             ptrs = (_dev_ptr(matrix).value, _dev_ptr(lde_out).value, ctypes.c_void_p(layers_out.data_ptr()).value)
         else:
             ptrs = (_host_u64(matrix).ctypes.data, _host_u64(lde_out).ctypes.data, layers_out.ctypes.data)
-        job = _Job(GL_JOB_COMMIT, int(dev), ptrs[0], rows, cols, blowup, ptrs[1], ptrs[2])
+        job = _Job(GL_JOB_COMMIT_ROWS if row_tree else GL_JOB_COMMIT, int(dev), ptrs[0], rows, cols, blowup,
+                   ptrs[1], ptrs[2])
         return self._submit(job, (matrix, lde_out, layers_out))
 
     def poll(self, ticket):

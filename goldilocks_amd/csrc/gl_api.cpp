@@ -34,6 +34,53 @@ size_t tree_size(size_t len) {
 
 hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 
+// Row tree over `leaves` row hashes: nodes of every layer, leaves included.
+size_t row_tree_nodes(size_t leaves) {
+    size_t total = leaves;
+    while (leaves > 1) {
+        leaves = (leaves + 1) / 2;
+        total += leaves;
+    }
+    return total;
+}
+
+int row_geometry(size_t leaves, RowGeometry &g) {
+    g.depth = 0;
+    g.n[0] = leaves;
+    g.off[0] = 0;
+    u64 n = leaves, off = 0;
+    while (n > 1) {
+        if (g.depth == kMaxRowDepth) return set_error(GL_E_UNSUPPORTED, "row tree too deep");
+        off += n;
+        n = (n + 1) / 2;
+        g.depth++;
+        g.n[g.depth] = n;
+        g.off[g.depth] = off;
+    }
+    return GL_OK;
+}
+
+int row_tree_check(size_t nrows, size_t cols) {
+    if (nrows == 0 || cols == 0) return set_error(GL_E_ARG, "empty row tree");
+    if (cols > kRowTreeMaxCols)
+        return set_error(GL_E_SIZE, "row tree: rows of at most 1023 words (a single K12 node)");
+    return GL_OK;
+}
+
+int row_tree_dev_impl(const u64 *d_rows, size_t nrows, size_t cols, uint8_t *d_out, size_t cap, size_t *out_len,
+                      hipStream_t s) {
+    int rc = row_tree_check(nrows, cols);
+    if (rc) return rc;
+    const size_t need = 32 * row_tree_nodes(nrows);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the row tree");
+    if (!d_rows || !d_out) return set_error(GL_E_ARG, "null device pointer");
+    if ((uintptr_t)d_rows % 8 || (uintptr_t)d_out % 8) return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
+    launch_row_tree(d_rows, nrows, (u32)cols, (u64 *)d_out, s);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
 int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) {
     int rc = ensure_init();
     if (rc) return rc;
@@ -82,15 +129,18 @@ int merkle_dev_impl(const uint8_t *d_in, size_t len, uint8_t *d_out, size_t cap,
 
 namespace gl {
 
-int commit_check(size_t rows, size_t cols, unsigned blowup) {
+int commit_check(size_t rows, size_t cols, unsigned blowup, CommitTree tree) {
     const u64 ext = (u64)rows * blowup;
     if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
     if (!((P - 1) % rows == 0) || !((P - 1) % ext == 0))
         return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
+    if (tree == TREE_ROWS && cols > kRowTreeMaxCols)
+        return set_error(GL_E_SIZE, "row-tree commit: rows of at most 1023 words (a single K12 node)");
     return GL_OK;
 }
 
-size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup) {
+size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup, CommitTree tree) {
+    if (tree == TREE_ROWS) return 32 * row_tree_nodes((u64)rows * blowup);
     return tree_size((u64)rows * blowup * cols * 8);
 }
 
@@ -119,7 +169,7 @@ int commit_prepare(size_t rows, size_t cols, unsigned blowup, Keep *keep) {
 // Enqueue the whole commit on stream s (no host synchronization):
 // coef (rows*cols words) and tmp (blowup*rows*cols words) are scratch.
 int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
-                   u64 *coef, u64 *tmp, hipStream_t s) {
+                   u64 *coef, u64 *tmp, hipStream_t s, CommitTree tree) {
     const u64 ext = (u64)rows * blowup;
     PlanRef prow, pext;
     int rc;
@@ -156,6 +206,7 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
         GL_HIP(hipGetLastError());
         if ((rc = run_plan(*pext, d_lde, tmp, cols, 1, ext * cols, false, s))) return rc;
     }
+    if (tree == TREE_ROWS) return row_tree_dev_impl(d_lde, ext, cols, d_layers, commit_tree_size(rows, cols, blowup, tree), nullptr, s);
     size_t need = tree_size(ext * cols * 8);
     return merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, need, nullptr, s);
 }
@@ -534,46 +585,148 @@ int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
     return GL_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------- commit
+
+namespace {
+
+int commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde, uint8_t *d_layers,
+               size_t cap, size_t *out_len, hipStream_t s, CommitTree tree) {
+    int rc;
+    if ((rc = commit_check(rows, cols, blowup, tree))) return rc;
+    const u64 ext = (u64)rows * blowup;
+    size_t need = commit_tree_size(rows, cols, blowup, tree);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the commit's tree");
+    if (!d_m || !d_lde || !d_layers) return set_error(GL_E_ARG, "null device pointer");
+    if (tree == TREE_ROWS && (uintptr_t)d_layers % 8) return set_error(GL_E_ARG, "row tree output must be 8-byte aligned");
+    // scratch returns to the pool stream ordered: no wait needed
+    DevBuf coef(rows * cols * 8, s);
+    DevBuf tmp(ext * cols * 8, s);
+    if (!coef.p || !tmp.p) return set_error(GL_E_NOMEM, "commit scratch allocation failed");
+    return commit_enqueue(d_m, rows, cols, blowup, d_lde, d_layers, coef.as<u64>(), tmp.as<u64>(), s, tree);
+}
+
+int commit_host(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out, uint8_t *layers_out,
+                size_t cap, size_t *out_len, CommitTree tree) {
+    int rc;
+    if ((rc = commit_check(rows, cols, blowup, tree))) return rc;
+    const u64 ext = (u64)rows * blowup;
+    size_t need = commit_tree_size(rows, cols, blowup, tree);
+    if (out_len) *out_len = need;
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the commit's tree");
+    if (!m || !lde_out || !layers_out) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf dm(rows * cols * 8, s), dl(ext * cols * 8, s), dt(need + 256, s);
+    if (!dm.p || !dl.p || !dt.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    GL_HIP(hipMemcpyAsync(dm.p, m, rows * cols * 8, hipMemcpyHostToDevice, s));
+    if ((rc = commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need, nullptr, s, tree)))
+        return rc;
+    GL_HIP(hipMemcpyAsync(lde_out, dl.p, ext * cols * 8, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipMemcpyAsync(layers_out, dt.p, need, hipMemcpyDeviceToHost, s));
+    GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde,
                   uint8_t *d_layers, size_t cap, size_t *out_len, void *stream) {
     int rc = ensure_init();
     if (rc) return rc;
-    hipStream_t s = as_stream(stream);
-    if ((rc = commit_check(rows, cols, blowup))) return rc;
-    const u64 ext = (u64)rows * blowup;
-    size_t need = tree_size(ext * cols * 8);
-    if (out_len) *out_len = need;
-    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
-    if (!d_m || !d_lde || !d_layers) return set_error(GL_E_ARG, "null device pointer");
-    // scratch returns to the pool stream ordered: no wait needed
-    DevBuf coef(rows * cols * 8, s);
-    DevBuf tmp(ext * cols * 8, s);
-    if (!coef.p || !tmp.p) return set_error(GL_E_NOMEM, "commit scratch allocation failed");
-    return commit_enqueue(d_m, rows, cols, blowup, d_lde, d_layers, coef.as<u64>(), tmp.as<u64>(), s);
+    return commit_dev(d_m, rows, cols, blowup, d_lde, d_layers, cap, out_len, as_stream(stream), TREE_BYTES);
 }
 
 int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
               uint8_t *layers_out, size_t cap, size_t *out_len) {
     int rc = ensure_init();
     if (rc) return rc;
-    if ((rc = commit_check(rows, cols, blowup))) return rc;
-    const u64 ext = (u64)rows * blowup;
-    size_t need = tree_size(ext * cols * 8);
+    return commit_host(m, rows, cols, blowup, lde_out, layers_out, cap, out_len, TREE_BYTES);
+}
+
+int gl_commit_rows_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde,
+                       uint8_t *d_layers, size_t cap, size_t *out_len, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    return commit_dev(d_m, rows, cols, blowup, d_lde, d_layers, cap, out_len, as_stream(stream), TREE_ROWS);
+}
+
+int gl_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
+                   uint8_t *layers_out, size_t cap, size_t *out_len) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    return commit_host(m, rows, cols, blowup, lde_out, layers_out, cap, out_len, TREE_ROWS);
+}
+
+// ---------------------------------------------------------------- row tree
+
+size_t gl_row_tree_size(size_t leaves) { return 32 * row_tree_nodes(leaves); }
+
+int gl_row_tree_dev(const uint64_t *d_rows, size_t nrows, size_t cols, uint8_t *d_layers, size_t cap,
+                    size_t *out_len, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    return row_tree_dev_impl(d_rows, nrows, cols, d_layers, cap, out_len, as_stream(stream));
+}
+
+int gl_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *layers_out, size_t cap, size_t *out_len) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = row_tree_check(nrows, cols))) return rc;
+    const size_t need = gl_row_tree_size(nrows);
     if (out_len) *out_len = need;
-    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the Merkle tree");
-    if (!m || !lde_out || !layers_out) return set_error(GL_E_ARG, "null pointer");
+    if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the row tree");
+    if (!rows || !layers_out) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = thread_stream();
-    DevBuf dm(rows * cols * 8, s), dl(ext * cols * 8, s), dt(need + 256, s);
-    if (!dm.p || !dl.p || !dt.p) return set_error(GL_E_NOMEM, "device allocation failed");
-    GL_HIP(hipMemcpyAsync(dm.p, m, rows * cols * 8, hipMemcpyHostToDevice, s));
-    if ((rc = gl_commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need,
-                            nullptr, s)))
-        return rc;
-    GL_HIP(hipMemcpyAsync(lde_out, dl.p, ext * cols * 8, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipMemcpyAsync(layers_out, dt.p, need, hipMemcpyDeviceToHost, s));
+    const size_t in_bytes = nrows * cols * 8, in_pad = (in_bytes + 255) / 256 * 256;
+    DevBuf d(in_pad + need, s);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
+    GL_HIP(hipMemcpyAsync(din, rows, in_bytes, hipMemcpyHostToDevice, s));
+    if ((rc = row_tree_dev_impl((const u64 *)din, nrows, cols, dout, need, nullptr, s))) return rc;
+    GL_HIP(hipMemcpyAsync(layers_out, dout, need, hipMemcpyDeviceToHost, s));
     GL_HIP(hipStreamSynchronize(s));
+    return GL_OK;
+}
+
+size_t gl_row_tree_depth(size_t leaves) {
+    size_t d = 0;
+    for (size_t n = leaves; n > 1; n = (n + 1) / 2) d++;
+    return d;
+}
+
+int gl_row_open_dev(const uint8_t *d_layers, size_t leaves, const uint64_t *d_idx, size_t k, uint8_t *d_out,
+                    void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    RowGeometry g;
+    if (leaves == 0) return set_error(GL_E_ARG, "empty row tree");
+    if ((rc = row_geometry(leaves, g))) return rc;
+    if (k == 0 || g.depth == 0) return GL_OK;
+    if (!d_layers || !d_idx || !d_out) return set_error(GL_E_ARG, "null device pointer");
+    if ((uintptr_t)d_layers % 8 || (uintptr_t)d_out % 8) return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
+    launch_row_open(g, (const u64 *)d_layers, d_idx, k, (u64 *)d_out, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int gl_row_verify_dev(const uint64_t *d_rows, size_t cols, size_t leaves, const uint64_t *d_idx, size_t k,
+                      const uint8_t *d_paths, const uint8_t *d_root32, int32_t *d_ok, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = row_tree_check(leaves, cols))) return rc;
+    RowGeometry g;
+    if ((rc = row_geometry(leaves, g))) return rc;
+    if (k == 0) return GL_OK;
+    if (!d_rows || !d_idx || !d_root32 || !d_ok || (g.depth && !d_paths))
+        return set_error(GL_E_ARG, "null device pointer");
+    if ((uintptr_t)d_paths % 8 || (uintptr_t)d_root32 % 8) return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
+    launch_row_verify(g, d_rows, (u32)cols, d_idx, k, (const u64 *)d_paths, (const u64 *)d_root32, (int *)d_ok,
+                      as_stream(stream));
+    GL_HIP(hipGetLastError());
     return GL_OK;
 }
 

@@ -1,4 +1,4 @@
-// Keccak-p[1600,12] / single-block KangarooTwelve layer hashing for gfx950.
+// Keccak-p[1600,12] / KangarooTwelve layer hashing and the row tree for gfx950.
 //
 // One 25-lane state per thread (50 VGPRs): a layer of the reference's Merkle
 // tree (pcs/src/k12/mod.rs:41-60) hashes independent 160-byte blocks, so each
@@ -247,6 +247,188 @@ bool launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage_in, b
     hipLaunchKernelGGL(k12_layer_kernel, dim3(grid), dim3(256), 0, s, (const u64 *)in, nfull, nblocks,
                        (const u64 *)stage_in, (u64 *)out, produce ? stage_out : nullptr, next_first);
     return produce;
+}
+
+// ---- Row tree: the binary tree of row hashes (no reference function).
+// Leaf r = K12(row r): the row's 8 * cols bytes and the empty customization's
+// length byte 0x00, absorbed 21 words (168 bytes) at a time by Keccak-p[1600,
+// 12] with suffix 0x07 and 0x80 at byte 167 of the last block (TurboSHAKE128,
+// D = 0x07: K12's single node, rows up to kRowTreeMaxCols words).  The word
+// after the data holds the bytes 0x00, 0x07: the constant 0x0700.  A parent
+// is K12(left || right), 64 bytes in one permutation; the last node of an
+// odd layer moves up unchanged.
+
+// Absorb and permute row w (cols words) into a[0..3] = its 32-byte hash.
+__device__ __forceinline__ void row_leaf(const u64 *w, u32 cols, u64 *a) {
+#pragma unroll
+    for (int i = 0; i < 25; i++) a[i] = 0;
+    const u32 nb = cols / 21 + 1;
+    for (u32 b = 0; b + 1 < nb; b++) {
+        const u64 *blk = w + 21 * b;
+#pragma unroll
+        for (int i = 0; i < 21; i++) a[i] ^= blk[i];
+        keccak_p12(a);
+    }
+    const u32 base = 21 * (nb - 1);
+#pragma unroll
+    for (int i = 0; i < 21; i++) {
+        const u32 j = base + i;
+        a[i] ^= j < cols ? w[j] : (j == cols ? 0x0700ull : 0);
+    }
+    a[20] ^= 0x8000000000000000ull;
+    keccak_p12(a);
+}
+
+// h = K12(l || r) for 32-byte l, r.
+__device__ __forceinline__ void pair_hash(const u64 *l, const u64 *r, u64 *h) {
+    u64 a[25];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        a[i] = l[i];
+        a[4 + i] = r[i];
+    }
+    a[8] = 0x0700ull;
+#pragma unroll
+    for (int i = 9; i < 25; i++) a[i] = 0;
+    a[20] = 0x8000000000000000ull;
+    keccak_p12(a);
+#pragma unroll
+    for (int i = 0; i < 4; i++) h[i] = a[i];
+}
+
+__global__ __launch_bounds__(256) void k12_rows_kernel(const u64 *__restrict__ rows, u64 nrows, u32 cols,
+                                                       u64 *__restrict__ out) {
+    for (u64 r = blockIdx.x * (u64)blockDim.x + threadIdx.x; r < nrows; r += (u64)gridDim.x * blockDim.x) {
+        u64 a[25];
+        row_leaf(rows + r * cols, cols, a);
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[4 * r + i] = a[i];
+    }
+}
+
+// Layer of n nodes (4 words each) -> its ceil(n / 2) parents.
+__global__ __launch_bounds__(256) void k12_pairs_kernel(const u64 *__restrict__ in, u64 n, u64 *__restrict__ out) {
+    const u64 m = (n + 1) / 2;
+    for (u64 j = blockIdx.x * (u64)blockDim.x + threadIdx.x; j < m; j += (u64)gridDim.x * blockDim.x) {
+        u64 h[4];
+        if (2 * j + 1 < n) {
+            pair_hash(in + 8 * j, in + 8 * j + 4, h);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) h[i] = in[8 * j + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[4 * j + i] = h[i];
+    }
+}
+
+// The last layers in one workgroup: a layer of n <= kRowTail nodes is read
+// into LDS once and every layer above it is built there (and written out)
+// up to the root.  Every thread runs the same layer loop (n is uniform), so
+// each barrier is reached by the whole workgroup.
+constexpr u32 kRowTail = 512;
+__global__ __launch_bounds__(256) void k12_pairs_tail_kernel(const u64 *__restrict__ in, u64 n,
+                                                             u64 *__restrict__ out) {
+    __shared__ u64 buf[2][kRowTail * 4];
+    for (u32 i = threadIdx.x; i < 4 * n; i += blockDim.x) buf[0][i] = in[i];
+    __syncthreads();
+    int cur = 0;
+    while (n > 1) {
+        const u64 m = (n + 1) / 2;
+        for (u32 j = threadIdx.x; j < m; j += blockDim.x) {
+            const u64 *src = buf[cur];
+            u64 h[4];
+            if (2 * j + 1 < n) {
+                pair_hash(src + 8 * j, src + 8 * j + 4, h);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) h[i] = src[8 * j + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                buf[1 - cur][4 * j + i] = h[i];
+                out[4 * j + i] = h[i];
+            }
+        }
+        __syncthreads();
+        out += 4 * m;
+        n = m;
+        cur = 1 - cur;
+    }
+}
+
+void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, hipStream_t s) {
+    if (nrows == 0) return;
+    auto grid_for = [](u64 threads) { u64 g = (threads + 255) / 256; return (unsigned)(g < 65536 ? g : 65536); };
+    hipLaunchKernelGGL(k12_rows_kernel, dim3(grid_for(nrows)), dim3(256), 0, s, rows, nrows, cols, out);
+    u64 n = nrows;
+    u64 *layer = out;
+    while (n > kRowTail) {
+        hipLaunchKernelGGL(k12_pairs_kernel, dim3(grid_for((n + 1) / 2)), dim3(256), 0, s, (const u64 *)layer, n,
+                           layer + 4 * n);
+        layer += 4 * n;
+        n = (n + 1) / 2;
+    }
+    if (n > 1) hipLaunchKernelGGL(k12_pairs_tail_kernel, dim3(1), dim3(256), 0, s, (const u64 *)layer, n, layer + 4 * n);
+}
+
+// Opening i of leaf idx[i]: for layer l < depth, the sibling of node
+// idx[i] >> l (4 words), or zeros where that node has none (it moves up
+// unchanged) or idx[i] is not a leaf.  One thread per (opening, layer).
+__global__ void row_open_kernel(RowGeometry g, const u64 *__restrict__ layers, const u64 *__restrict__ idx, u64 k,
+                                u64 *__restrict__ out) {
+    const u64 total = k * (u64)g.depth;
+    for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
+        const u64 i = t / g.depth, l = t % g.depth;
+        const u64 leaf = idx[i];
+        const u64 sib = (leaf >> l) ^ 1;
+        const bool has = leaf < g.n[0] && sib < g.n[l];
+#pragma unroll
+        for (int w = 0; w < 4; w++) out[4 * t + w] = has ? layers[4 * (g.off[l] + sib) + w] : 0;
+    }
+}
+
+// ok[i] = 1 when row rows[i] (cols words) at leaf idx[i] with opening
+// paths[i] hashes up to root.
+__global__ __launch_bounds__(64) void row_verify_kernel(RowGeometry g, const u64 *__restrict__ rows, u32 cols,
+                                                        const u64 *__restrict__ idx, u64 k,
+                                                        const u64 *__restrict__ paths,
+                                                        const u64 *__restrict__ root, int *__restrict__ ok) {
+    for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < k; i += (u64)gridDim.x * blockDim.x) {
+        u64 node = idx[i];
+        if (node >= g.n[0]) {
+            ok[i] = 0;
+            continue;
+        }
+        u64 a[25];
+        row_leaf(rows + i * cols, cols, a);
+        u64 h[4] = {a[0], a[1], a[2], a[3]};
+        for (int l = 0; l < g.depth; l++) {
+            const u64 *sib = paths + 4 * (i * (u64)g.depth + l);
+            if ((node ^ 1) < g.n[l]) {
+                if (node & 1) pair_hash(sib, h, h);
+                else pair_hash(h, sib, h);
+            }
+            node >>= 1;
+        }
+        ok[i] = (h[0] == root[0] && h[1] == root[1] && h[2] == root[2] && h[3] == root[3]) ? 1 : 0;
+    }
+}
+
+void launch_row_open(const RowGeometry &g, const u64 *layers, const u64 *idx, u64 k, u64 *out, hipStream_t s) {
+    const u64 total = k * (u64)g.depth;
+    if (total == 0) return;
+    u64 grid = (total + 255) / 256;
+    hipLaunchKernelGGL(row_open_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(256), 0, s, g, layers, idx,
+                       k, out);
+}
+
+void launch_row_verify(const RowGeometry &g, const u64 *rows, u32 cols, const u64 *idx, u64 k, const u64 *paths,
+                       const u64 *root, int *ok, hipStream_t s) {
+    if (k == 0) return;
+    u64 grid = (k + 63) / 64;
+    hipLaunchKernelGGL(row_verify_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(64), 0, s, g, rows, cols,
+                       idx, k, paths, root, ok);
 }
 
 // Proof-of-work scan (pcs/src/pow.rs:32-60): nonces [start, start+count) are

@@ -176,12 +176,16 @@ void timing_end(const PassVariant *v, hipEvent_t e0, hipStream_t s);
 size_t timing_pending();
 int timing_collect(uint32_t *ids, float *ms, size_t cap, size_t *count);
 
-// Commit pieces shared by the C ABI and the server (gl_api.cpp).
-int commit_check(size_t rows, size_t cols, unsigned blowup);
-size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup);
+// Commit pieces shared by the C ABI and the server (gl_api.cpp).  The tree
+// over the extension is the 5-ary byte Merkle tree (TREE_BYTES) or the
+// binary tree of row hashes (TREE_ROWS, at most kRowTreeMaxCols columns;
+// its layers output must be 8-byte aligned).
+enum CommitTree : int { TREE_BYTES = 0, TREE_ROWS = 1 };
+int commit_check(size_t rows, size_t cols, unsigned blowup, CommitTree tree = TREE_BYTES);
+size_t commit_tree_size(size_t rows, size_t cols, unsigned blowup, CommitTree tree = TREE_BYTES);
 // Plans and tables of a commit shape (built once; references in keep).
 int commit_prepare(size_t rows, size_t cols, unsigned blowup, Keep *keep);
 int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u64 *d_lde, uint8_t *d_layers,
-                   u64 *coef, u64 *tmp, hipStream_t s);
+                   u64 *coef, u64 *tmp, hipStream_t s, CommitTree tree = TREE_BYTES);
 
 }  // namespace gl

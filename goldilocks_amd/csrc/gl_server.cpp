@@ -67,10 +67,12 @@ size_t ntt_footprint(const gl_job &j, const Plan *plan) {
     return f;
 }
 
+CommitTree tree_of(const gl_job &j) { return j.kind == GL_JOB_COMMIT_ROWS ? TREE_ROWS : TREE_BYTES; }
+
 size_t commit_footprint(const gl_job &j) {
     size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
     size_t f = rows * cols * 8 + ext * cols * 8;  // coef + tmp
-    if (!j.device) f += rows * cols * 8 + ext * cols * 8 + commit_tree_size(rows, cols, j.blowup);
+    if (!j.device) f += rows * cols * 8 + ext * cols * 8 + commit_tree_size(rows, cols, j.blowup, tree_of(j));
     return f;
 }
 
@@ -131,7 +133,7 @@ int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
         if (!j.device) GL_HIP(hipMemcpyAsync(j.data, data, words * 8, hipMemcpyDeviceToHost, s));
     } else {
         size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
-        size_t tree = commit_tree_size(rows, cols, j.blowup);
+        size_t tree = commit_tree_size(rows, cols, j.blowup, tree_of(j));
         const u64 *m = j.data;
         u64 *lde = j.lde_out;
         uint8_t *layers = j.layers_out;
@@ -145,7 +147,7 @@ int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
         }
         u64 *coef = (u64 *)alloc(rows * cols * 8), *tmp = (u64 *)alloc(ext * cols * 8);
         if (!coef || !tmp) return set_error(GL_E_NOMEM, "server: allocation failed");
-        if ((rc = commit_enqueue(m, rows, cols, j.blowup, lde, layers, coef, tmp, s))) return rc;
+        if ((rc = commit_enqueue(m, rows, cols, j.blowup, lde, layers, coef, tmp, s, tree_of(j)))) return rc;
         if (!j.device) {
             GL_HIP(hipMemcpyAsync(j.lde_out, lde, ext * cols * 8, hipMemcpyDeviceToHost, s));
             if (tree) GL_HIP(hipMemcpyAsync(j.layers_out, layers, tree, hipMemcpyDeviceToHost, s));
@@ -181,7 +183,8 @@ void gl_server::finish_job(Slot &slot, int status) {
         else latencies[lat_next] = lat;
         lat_next = (lat_next + 1) % kLatencyWindow;
         const gl_job &j = job->spec;
-        elements += j.kind == GL_JOB_COMMIT ? (double)j.n * j.blowup * j.count : (double)j.n * j.count;
+        elements += j.kind == GL_JOB_NTT || j.kind == GL_JOB_INTT ? (double)j.n * j.count
+                                                                  : (double)j.n * j.blowup * j.count;
     } else {
         failed++;
     }
@@ -281,10 +284,12 @@ int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket) {
             if (rc) return rc;
         }
         j->footprint = ntt_footprint(s, j->plan.get());
-    } else if (s.kind == GL_JOB_COMMIT) {
-        int rc = commit_check(s.n, s.count, s.blowup);
+    } else if (s.kind == GL_JOB_COMMIT || s.kind == GL_JOB_COMMIT_ROWS) {
+        int rc = commit_check(s.n, s.count, s.blowup, tree_of(s));
         if (rc) return rc;
         if (!s.data || !s.lde_out || !s.layers_out) return set_error(GL_E_ARG, "server: null commit buffer");
+        if (s.device && s.kind == GL_JOB_COMMIT_ROWS && (uintptr_t)s.layers_out % 8)
+            return set_error(GL_E_ARG, "server: row tree output must be 8-byte aligned");
         if ((rc = commit_prepare(s.n, s.count, s.blowup, &j->keep))) return rc;
         j->footprint = commit_footprint(s);
     } else {

@@ -243,6 +243,41 @@ int gl_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint
 int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup,
                   uint64_t *d_lde, uint8_t *d_layers, size_t cap, size_t *out_len, void *stream);
 
+/* ---- Row tree and row-tree commit (no reference function: the binary tree
+ * of row hashes, an alternative to merkle_tree over the commit's bytes) -----
+ * Leaf r = KangarooTwelve of row r's 8*cols little-endian bytes (empty
+ * customization, 32-byte output; rows of at most 1023 words, K12's single
+ * node, GL_E_SIZE beyond): TurboSHAKE128 of the row and 0x00 with D = 0x07,
+ * several Keccak-p[1600,12] blocks per row (the same permutation, padding
+ * and constants as k12::<32>, pcs/src/k12/generic.rs:62-84, which is this
+ * for inputs up to 166 bytes).  A layer of n nodes gives ceil(n/2) parents,
+ * parent j = k12::<32>(node 2j || node 2j+1); the last node of an odd layer
+ * moves up unchanged; layers until one node, the root.  Layers output: every
+ * layer in order, leaves first, 32 bytes per node, gl_row_tree_size(leaves)
+ * bytes; device outputs must be 8-byte aligned. */
+size_t gl_row_tree_size(size_t leaves);
+/* Layers above the leaves (ceil(log2(leaves))). */
+size_t gl_row_tree_depth(size_t leaves);
+int gl_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *layers_out, size_t cap, size_t *out_len);
+int gl_row_tree_dev(const uint64_t *d_rows, size_t nrows, size_t cols, uint8_t *d_layers, size_t cap,
+                    size_t *out_len, void *stream);
+/* The commit's column coset extension (as gl_commit) with the row tree over
+ * its rows; layers_out holds gl_row_tree_size(blowup*rows) bytes. */
+int gl_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out,
+                   uint8_t *layers_out, size_t cap, size_t *out_len);
+int gl_commit_rows_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup, uint64_t *d_lde,
+                       uint8_t *d_layers, size_t cap, size_t *out_len, void *stream);
+/* k openings of leaves d_idx[i] into d_out, k * depth * 32 bytes: for layer
+ * l < depth the sibling of node d_idx[i] >> l, or 32 zero bytes where that
+ * node has none (it moves up unchanged) or d_idx[i] >= leaves. */
+int gl_row_open_dev(const uint8_t *d_layers, size_t leaves, const uint64_t *d_idx, size_t k, uint8_t *d_out,
+                    void *stream);
+/* d_ok[i] = 1 when row i of d_rows (k x cols words: the opened rows' values)
+ * at leaf d_idx[i] with opening i (layout of gl_row_open_dev) hashes up to
+ * d_root32, else 0. */
+int gl_row_verify_dev(const uint64_t *d_rows, size_t cols, size_t leaves, const uint64_t *d_idx, size_t k,
+                      const uint8_t *d_paths, const uint8_t *d_root32, int32_t *d_ok, void *stream);
+
 /* ---- Captured batch plans (hipGraph) -------------------------------------
  * A mixed batch over fixed device buffers, planned once: descriptors are
  * uploaded at creation and the launch sequence is captured as a hipGraph, so
@@ -264,9 +299,10 @@ void gl_batch_plan_destroy(gl_batch_plan *plan);
 #define GL_JOB_NTT 0
 #define GL_JOB_INTT 1
 #define GL_JOB_COMMIT 2
+#define GL_JOB_COMMIT_ROWS 3  /* gl_commit_rows: layers_out holds gl_row_tree_size(blowup*rows) bytes */
 
 typedef struct {
-    int kind;             /* GL_JOB_NTT / GL_JOB_INTT / GL_JOB_COMMIT */
+    int kind;             /* GL_JOB_NTT / GL_JOB_INTT / GL_JOB_COMMIT / GL_JOB_COMMIT_ROWS */
     int device;           /* nonzero: all buffers below are device pointers */
     uint64_t *data;       /* NTT: count*n words, transformed in place; COMMIT: rows x cols input */
     size_t n;             /* NTT length, or commit rows */

@@ -8,7 +8,8 @@
  * intt (ntt/src/ntt/mod.rs:104-120), coefficient scaling by 2717^i
  * (GENERATOR, ntt/src/field/algo/mod.rs:12), ntt (mod.rs:98-102), the raw
  * little-endian byte view of a Field slice (bench/src/main.rs:179-183) and
- * merkle_tree.
+ * merkle_tree.  or_commit_rows replaces the last step by the binary tree of
+ * row hashes (or_row_tree).
  */
 #include "oracle.h"
 
@@ -38,8 +39,8 @@ static void commit_column(void *arg, size_t c) {
     free(col);
 }
 
-int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
-              uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
+/* The column-wise coset extension into lde_out; -1 for unsupported shapes. */
+static int lde(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, uint64_t *lde_out) {
     size_t ext = rows * (size_t)blowup;
     if (rows == 0 || cols == 0 || blowup == 0 || !or_is_divisor(rows) || !or_is_divisor(ext))
         return -1;
@@ -47,8 +48,20 @@ int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
     /* columns are independent: one per thread (each column's transforms
      * then run serially inside the parallel loop) */
     or_par_for(cols, ext * 64, commit_column, &cx);
+    return 0;
+}
+
+int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
+              uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
+    if (lde(m, rows, cols, blowup, lde_out)) return -1;
     /* Field words are little-endian u64, so the byte view is the memory. */
-    return or_merkle_tree((const uint8_t *)lde_out, ext * cols * 8, layers_out, cap, out_len);
+    return or_merkle_tree((const uint8_t *)lde_out, rows * (size_t)blowup * cols * 8, layers_out, cap, out_len);
+}
+
+int or_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
+                   uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
+    if (cols > 1023 || lde(m, rows, cols, blowup, lde_out)) return -1;
+    return or_row_tree(lde_out, rows * (size_t)blowup, cols, layers_out, cap, out_len);
 }
 
 static uint64_t splitmix64(uint64_t *state) {

@@ -1,6 +1,8 @@
 /*
  * Oracle: Keccak-p[1600,12], single-block KangarooTwelve, layer hashing and
- * the 5-ary byte Merkle tree, plus the PoW check built on them.
+ * the 5-ary byte Merkle tree, plus the PoW check built on them; the Keccak
+ * sponge, single-node KangarooTwelve of longer messages and the binary row
+ * tree (no reference function) that the row-tree commit uses.
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Restates pcs/src/k12/generic.rs:4-94, pcs/src/k12/mod.rs:28-60 and
@@ -84,32 +86,55 @@ int or_k12(const uint8_t *in, size_t len, uint8_t *out, size_t outlen) {
     return 0;
 }
 
-/* SHA3-256 (24 rounds, rate 136, suffix 0x06): used only to pin the
- * permutation against an independent library implementation. */
-void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]) {
-    uint64_t st[25] = {0};
+/* The Keccak sponge over Keccak-p[1600, rounds] with byte-granular input:
+ * absorb `rate`-byte blocks, then the last partial block followed by the
+ * byte `suffix` (domain bits plus the first padding bit), zeros, and 0x80
+ * XORed into byte rate - 1 (pad10*1); squeeze outlen <= rate bytes.
+ * SHA3-256 is (rate 136, suffix 0x06, 24 rounds) and SHAKE128 (168, 0x1f,
+ * 24) in FIPS 202, which pins this function against hashlib; single-node
+ * KangarooTwelve is (168, 0x07, 12) over M || 0x00 (the KangarooTwelve
+ * specification, RFC 9861 section 3). */
+int or_keccak_sponge(const uint8_t *in, size_t len, size_t rate, uint8_t suffix, int rounds,
+                     uint8_t *out, size_t outlen) {
+    if (rate == 0 || rate > 200 || rate % 8 != 0 || outlen > rate || rounds < 1 || rounds > 24) return -1;
+    uint64_t st[25] = {0}, t[25];
     uint8_t block[200];
-    const size_t rate = 136;
-    while (len >= rate) {
+    for (;;) {
+        const size_t take = len < rate ? len : rate;
         memset(block, 0, sizeof block);
-        memcpy(block, in, rate);
-        uint64_t t[25];
+        memcpy(block, in, take);
+        const int last = len < rate;
+        if (last) {
+            block[take] ^= suffix;
+            block[rate - 1] ^= 0x80;
+        }
         load_state(t, block);
-        for (int i = 0; i < 17; i++) st[i] ^= t[i];
-        or_keccak_p1600(st, 24);
+        for (size_t i = 0; i < rate / 8; i++) st[i] ^= t[i];
+        or_keccak_p1600(st, rounds);
+        if (last) break;
         in += rate;
         len -= rate;
     }
-    memset(block, 0, sizeof block);
-    memcpy(block, in, len);
-    block[len] ^= 0x06;
-    block[rate - 1] ^= 0x80;
-    uint64_t t[25];
-    load_state(t, block);
-    for (int i = 0; i < 17; i++) st[i] ^= t[i];
-    or_keccak_p1600(st, 24);
     store_state(st, block);
-    memcpy(out, block, 32);
+    memcpy(out, block, outlen);
+    return 0;
+}
+
+/* SHA3-256: used only to pin the permutation against a library. */
+void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]) {
+    or_keccak_sponge(in, len, 136, 0x06, 24, out, 32);
+}
+
+/* KangarooTwelve with the empty customization string for messages that fit
+ * its single node, len + 1 <= 8192 bytes: TurboSHAKE128(M || 0x00, D = 0x07)
+ * (RFC 9861 sections 2-3).  For len <= 166 this is or_k12 (generic.rs:62-84,
+ * one permutation); longer messages absorb several 168-byte blocks. */
+int or_k12_long(const uint8_t *in, size_t len, uint8_t *out, size_t outlen) {
+    if (len > 8191 || outlen > 168) return -1;
+    uint8_t msg[8192];
+    memcpy(msg, in, len);
+    msg[len] = 0x00; /* length_encode(0) of the customization string */
+    return or_keccak_sponge(msg, len + 1, 168, 0x07, 12, out, outlen);
 }
 
 #define BLOCK_SIZE 160
@@ -163,6 +188,65 @@ int or_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
         layer = dst;
         len = or_next_layer_size(len);
         dst += len;
+    }
+    return 0;
+}
+
+/* ---- Row tree: the binary tree of row hashes (no reference function).
+ * Leaf r = K12(row r), the row's 8 * cols little-endian bytes (cols <= 1023,
+ * K12's single node); a layer of n nodes gives ceil(n / 2) nodes, node j =
+ * K12(node 2j || node 2j + 1) (64 bytes, one permutation), the last node of
+ * an odd layer moving up unchanged; layers until one node, the root.  The
+ * output is every layer in order, leaves first, 32 bytes per node. */
+size_t or_row_tree_size(size_t leaves) {
+    if (leaves == 0) return 0;
+    size_t total = HASH_SIZE * leaves;
+    while (leaves > 1) {
+        leaves = (leaves + 1) / 2;
+        total += HASH_SIZE * leaves;
+    }
+    return total;
+}
+
+typedef struct {
+    const uint8_t *in;
+    size_t n, row_bytes;
+    uint8_t *out;
+} RowCtx;
+
+/* Leaves [64 i, 64 i + 64). */
+static void leaf_range(void *arg, size_t i) {
+    const RowCtx *cx = (const RowCtx *)arg;
+    for (size_t r = 64 * i; r < 64 * i + 64 && r < cx->n; r++)
+        or_k12_long(cx->in + r * cx->row_bytes, cx->row_bytes, cx->out + r * HASH_SIZE, HASH_SIZE);
+}
+
+/* Parents [64 i, 64 i + 64) of a layer of n nodes. */
+static void pair_range(void *arg, size_t i) {
+    const RowCtx *cx = (const RowCtx *)arg;
+    const size_t m = (cx->n + 1) / 2;
+    for (size_t j = 64 * i; j < 64 * i + 64 && j < m; j++) {
+        if (2 * j + 1 < cx->n)
+            or_k12(cx->in + 2 * j * HASH_SIZE, 2 * HASH_SIZE, cx->out + j * HASH_SIZE, HASH_SIZE);
+        else
+            memcpy(cx->out + j * HASH_SIZE, cx->in + 2 * j * HASH_SIZE, HASH_SIZE);
+    }
+}
+
+int or_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *out, size_t cap, size_t *out_len) {
+    const size_t need = or_row_tree_size(nrows);
+    if (out_len) *out_len = need;
+    if (nrows == 0 || cols == 0 || cols > 1023 || cap < need) return -1;
+    /* Field words are little-endian u64 on this host: the byte view is the memory */
+    RowCtx cx = {(const uint8_t *)rows, nrows, 8 * cols, out};
+    or_par_for((nrows + 63) / 64, 64 * (cols / 21 + 1) * 1024, leaf_range, &cx);
+    size_t n = nrows;
+    uint8_t *layer = out;
+    while (n > 1) {
+        RowCtx px = {layer, n, 0, layer + n * HASH_SIZE};
+        or_par_for(((n + 1) / 2 + 63) / 64, 64 * 1024, pair_range, &px);
+        layer += n * HASH_SIZE;
+        n = (n + 1) / 2;
     }
     return 0;
 }

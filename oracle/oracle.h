@@ -79,12 +79,26 @@ void or_par_for(size_t count, size_t work_per_item, void (*fn)(void *, size_t), 
 void or_keccak_p1600(uint64_t st[25], int rounds);  /* generic.rs:25-60 (rounds=12) */
 int or_k12(const uint8_t *in, size_t len, uint8_t *out, size_t outlen); /* generic.rs:62-84 */
 void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]); /* pinning aid */
+/* Keccak sponge, byte-granular padding (suffix byte, 0x80 at rate - 1);
+ * outlen <= rate.  Pinned: SHAKE128 / SHA3-256 parameters vs hashlib. */
+int or_keccak_sponge(const uint8_t *in, size_t len, size_t rate, uint8_t suffix, int rounds,
+                     uint8_t *out, size_t outlen);
+/* Single-node K12 (len <= 8191): equals or_k12 up to 166 bytes.  Pinned:
+ * or_k12 for len <= 166 and the XKCP KATs for 17^2 and 17^3 pattern bytes. */
+int or_k12_long(const uint8_t *in, size_t len, uint8_t *out, size_t outlen);
 size_t or_next_layer_size(size_t len);              /* mod.rs:33-35 */
 void or_process_layer(const uint8_t *in, size_t len, uint8_t *out); /* generic.rs:86-94 */
 /* All layers concatenated in order (mod.rs:41-60); *out_len = total bytes.
  * Returns -1 if cap is too small (then *out_len holds the required size). */
 int or_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size_t *out_len);
 size_t or_merkle_tree_size(size_t len);
+/* Row tree (no reference function; the optional binary tree of row hashes):
+ * leaf r = or_k12_long(row r bytes), node = or_k12(left || right), the last
+ * node of an odd layer moves up unchanged; every layer, leaves first.
+ * Returns -1 for cols > 1023, empty shapes or cap too small.  Pinned by a
+ * Python assembly of the layers over or_k12_long / or_k12. */
+int or_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *out, size_t cap, size_t *out_len);
+size_t or_row_tree_size(size_t leaves);
 int or_pow_verify(const uint8_t challenge[32], uint64_t threshold, uint64_t nonce); /* pow.rs:32-37 */
 uint64_t or_pow_solve(const uint8_t challenge[32], uint64_t threshold);            /* pow.rs:39-60 */
 
@@ -96,6 +110,9 @@ uint64_t or_pow_solve(const uint8_t challenge[32], uint64_t threshold);         
  * direct polynomial-evaluation check at tiny sizes. */
 int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
               uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len);
+/* The same extension with the row tree over the LDE rows (or_row_tree). */
+int or_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
+                   uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len);
 
 /* ---- seeded inputs ----------------------------------------------------------
  * Word i = the first splitmix64 output of the stream seeded with

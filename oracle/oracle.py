@@ -47,6 +47,10 @@ def lib():
             "or_keccak_p1600": (None, [vp, i32]),
             "or_k12": (i32, [cp, sz, vp, sz]),
             "or_sha3_256": (None, [cp, sz, vp]),
+            "or_keccak_sponge": (i32, [cp, sz, sz, ctypes.c_uint8, i32, vp, sz]),
+            "or_k12_long": (i32, [cp, sz, vp, sz]),
+            "or_row_tree": (i32, [vp, sz, sz, vp, sz, ctypes.POINTER(sz)]),
+            "or_row_tree_size": (sz, [sz]),
             "or_next_layer_size": (sz, [sz]),
             "or_process_layer": (None, [cp, sz, vp]),
             "or_merkle_tree": (i32, [cp, sz, vp, sz, ctypes.POINTER(sz)]),
@@ -54,6 +58,7 @@ def lib():
             "or_pow_verify": (i32, [cp, u64, u64]),
             "or_pow_solve": (u64, [cp, u64]),
             "or_commit": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, ctypes.POINTER(sz)]),
+            "or_commit_rows": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, ctypes.POINTER(sz)]),
             "or_fill_random": (None, [vp, sz, u64]),
         }
         for name, (res, args) in sigs.items():
@@ -145,6 +150,48 @@ def sha3_256(data: bytes) -> bytes:
     return out.raw
 
 
+def keccak_sponge(data: bytes, rate, suffix, rounds, outlen) -> bytes:
+    out = ctypes.create_string_buffer(max(1, outlen))
+    if lib().or_keccak_sponge(bytes(data), len(data), rate, suffix, rounds, out, outlen) != 0:
+        raise ValueError("bad sponge parameters")
+    return out.raw[:outlen]
+
+
+def k12_long(data: bytes, outlen=32) -> bytes:
+    """Single-node KangarooTwelve, messages up to 8191 bytes."""
+    out = ctypes.create_string_buffer(outlen)
+    if lib().or_k12_long(bytes(data), len(data), out, outlen) != 0:
+        raise ValueError("k12_long input longer than 8191 bytes or output longer than 168")
+    return out.raw
+
+
+def row_tree_size(leaves): return lib().or_row_tree_size(leaves)
+
+
+def _row_layers(raw, leaves):
+    layers, n, off = [], leaves, 0
+    while n >= 1:
+        layers.append(raw[off:off + 32 * n])
+        off += 32 * n
+        if n == 1:
+            break
+        n = (n + 1) // 2
+    return layers
+
+
+def row_tree(rows):
+    """Layers (leaves first, root last) of the binary tree of row hashes of a
+    row-major matrix of words."""
+    rows = _u64(rows)
+    nrows, cols = rows.shape
+    need = row_tree_size(nrows)
+    out = ctypes.create_string_buffer(max(1, need))
+    got = ctypes.c_size_t(0)
+    if lib().or_row_tree(_ptr(rows), nrows, cols, out, need, ctypes.byref(got)) != 0:
+        raise ValueError("unsupported row-tree shape")
+    return _row_layers(out.raw[:need], nrows)
+
+
 def next_layer_size(n): return lib().or_next_layer_size(n)
 
 
@@ -201,6 +248,20 @@ def commit(m, blowup):
         layers.append(raw[off:off + n])
         off += n
     return lde, layers
+
+
+def commit_rows(m, blowup):
+    """(lde, row-tree layers) of the row-tree commit."""
+    m = _u64(m)
+    rows, cols = m.shape
+    lde = np.zeros((rows * blowup, cols), dtype=np.uint64)
+    need = row_tree_size(rows * blowup)
+    out = ctypes.create_string_buffer(max(1, need))
+    got = ctypes.c_size_t(0)
+    rc = lib().or_commit_rows(_ptr(m), rows, cols, blowup, _ptr(lde), out, need, ctypes.byref(got))
+    if rc != 0:
+        raise ValueError("unsupported commit shape")
+    return lde, _row_layers(out.raw[:need], rows * blowup)
 
 
 def fill_random(n, seed):

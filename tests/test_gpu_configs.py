@@ -174,7 +174,8 @@ def test_tuned_schedule_cols(gl, oracle, n, sched):
 def test_commit_2_20_x_64_blowup_2(gl, oracle):
     """The commit workload itself: every LDE word and every node of every
     tree layer equal the oracle's (reference primitives composed as in
-    bench/src/main.rs:179-183 and pcs/src/k12/mod.rs:41-60)."""
+    bench/src/main.rs:179-183 and pcs/src/k12/mod.rs:41-60), and every node
+    of the row-tree form's 22 layers."""
     import torch
     rows, cols, blowup = 1 << 20, 64, 2
     note(f"commit {rows}x{cols} blowup {blowup}")
@@ -185,7 +186,13 @@ def test_commit_2_20_x_64_blowup_2(gl, oracle):
     host_m = from_torch(m).reshape(rows, cols)
     got_lde = from_torch(lde).reshape(rows * blowup, cols)
     got_layers = layers.cpu().numpy().tobytes()
-    del lde, layers, m
+    # the row-tree form of the same commit: the same extension, the binary
+    # tree of row hashes over it
+    note(f"commit_rows {rows}x{cols} blowup {blowup}")
+    lde_r, layers_r = gl.commit_rows(m, blowup)
+    assert torch.equal(lde_r, lde)
+    got_rows = layers_r.cpu().numpy().tobytes()
+    del lde, layers, m, lde_r, layers_r
     lde_o, layers_o = oracle.commit(host_m, blowup)
     bad = np.flatnonzero(got_lde.reshape(-1) != lde_o.reshape(-1))
     assert bad.size == 0, f"{bad.size} LDE words differ, first at {bad[:8]}"
@@ -196,6 +203,13 @@ def test_commit_2_20_x_64_blowup_2(gl, oracle):
         off += len(layer)
     assert off == len(got_layers)
     assert len(layers_o[-1]) == 32
+    rows_o = oracle.row_tree(lde_o)
+    assert len(rows_o) == 22 and len(rows_o[-1]) == 32
+    off = 0
+    for depth, layer in enumerate(rows_o):
+        assert got_rows[off:off + len(layer)] == layer, f"row tree layer {depth} differs"
+        off += len(layer)
+    assert off == len(got_rows)
 
 
 def test_batch_1024_mixed(gl, oracle):

@@ -84,3 +84,63 @@ def test_merkle_tree(oracle):
         prev = l
     assert oracle.merkle_tree(b"") == [] and oracle.merkle_tree(bytes(32)) == []
     assert len(oracle.merkle_tree(bytes(33))) == 1
+
+
+def test_sponge_vs_hashlib(oracle):
+    """The byte-granular sponge with FIPS 202's SHAKE128 and SHA3-256
+    parameters against hashlib, across the block boundaries."""
+    rng = random.Random(3)
+    lens = [0, 1, 166, 167, 168, 169, 335, 336, 337, 503, 504, 505, 1000] + [rng.randrange(1200) for _ in range(12)]
+    for n in lens:
+        msg = bytes(rng.randrange(256) for _ in range(n))
+        for outlen in (1, 32, 167, 168):
+            assert oracle.keccak_sponge(msg, 168, 0x1F, 24, outlen) == hashlib.shake_128(msg).digest(outlen), (n, outlen)
+        assert oracle.keccak_sponge(msg, 136, 0x06, 24, 32) == hashlib.sha3_256(msg).digest(), n
+    with pytest.raises(ValueError):
+        oracle.keccak_sponge(b"", 168, 0x1F, 24, 169)
+
+
+def test_k12_long(oracle):
+    rng = random.Random(4)
+    # one permutation: the reference-faithful single-block K12
+    for n in range(167):
+        msg = bytes(rng.randrange(256) for _ in range(n))
+        assert oracle.k12_long(msg, 32) == oracle.k12(msg, 32), n
+    assert oracle.k12_long(b"", 168) == oracle.k12(b"", 168)
+    # several blocks, single node (XKCP KangarooTwelve.txt; RFC 9861 section
+    # 5): the pattern repeats 0x00..0xFA, 251 values (the same bytes as the
+    # reference's (0..0xfa).cycle() only below 250)
+    ptn = lambda n: bytes(i % 251 for i in range(n))
+    assert oracle.k12_long(ptn(17 ** 2), 32) == hx(
+        "0c 31 5e bc de db f6 14 26 de 7d cf 8f b7 25 d1 e7 46 75 d7 f5 32 7a 50 67 f3 67 b1 08 ec b6 7c")
+    assert oracle.k12_long(ptn(17 ** 3), 32) == hx(
+        "cb 55 2e 2e c7 7d 99 10 70 1d 57 8b 45 7d df 77 2c 12 e3 22 e4 ee 7f e4 17 f9 2c 75 8f 0d 59 d0")
+    oracle.k12_long(bytes(8191), 32)
+    with pytest.raises(ValueError):
+        oracle.k12_long(bytes(8192), 32)
+
+
+def row_tree_py(oracle, rows):
+    """The row tree assembled here from the pinned hashes: leaf r =
+    k12_long(row bytes), parent = k12(left || right), odd last node moved up."""
+    layer = [oracle.k12_long(r.astype("<u8").tobytes(), 32) for r in rows]
+    layers = [b"".join(layer)]
+    while len(layer) > 1:
+        nxt = [oracle.k12(layer[2 * j] + layer[2 * j + 1], 32) if 2 * j + 1 < len(layer) else layer[2 * j]
+               for j in range((len(layer) + 1) // 2)]
+        layers.append(b"".join(nxt))
+        layer = nxt
+    return layers
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 64), (2, 3), (3, 20), (5, 21), (6, 22), (7, 41), (9, 42),
+                                   (33, 64), (100, 1023), (257, 5)])
+def test_row_tree(oracle, shape):
+    import numpy as np
+    rows = oracle.fill_random(shape[0] * shape[1], 11 + shape[0]).reshape(shape)
+    got = oracle.row_tree(rows)
+    assert got == row_tree_py(oracle, rows)
+    assert sum(len(l) for l in got) == oracle.row_tree_size(shape[0])
+    assert len(got[-1]) == 32
+    with pytest.raises(ValueError):
+        oracle.row_tree(np.zeros((4, 1024), dtype=np.uint64))
