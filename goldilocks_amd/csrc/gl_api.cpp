@@ -62,6 +62,7 @@ int row_geometry(size_t leaves, RowGeometry &g) {
 
 int row_tree_check(size_t nrows, size_t cols) {
     if (nrows == 0 || cols == 0) return set_error(GL_E_ARG, "empty row tree");
+    if (words_overflow(nrows, cols, 4)) return set_error(GL_E_ARG, "row tree shape overflows the address space");
     if (cols > kRowTreeMaxCols)
         return set_error(GL_E_SIZE, "row tree: rows of at most 1023 words (a single K12 node)");
     return GL_OK;
@@ -89,6 +90,7 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
     if ((rc = S0 > 1 ? get_plan_cols(n, &plan) : get_plan(n, &plan))) return rc;
     if (count == 0 || S0 == 0 || plan->passes.empty()) return GL_OK;
     if (!d) return set_error(GL_E_ARG, "null device pointer");
+    if (words_overflow(n, S0, count)) return set_error(GL_E_ARG, "n * count words overflow the address space");
     const u64 bstride = n * S0;
     const size_t words = plan_tmp_words(*plan, count, bstride);
     DevBuf tmp(words * 8, s);  // this call's scratch, stream ordered
@@ -132,6 +134,7 @@ namespace gl {
 int commit_check(size_t rows, size_t cols, unsigned blowup, CommitTree tree) {
     const u64 ext = (u64)rows * blowup;
     if (rows == 0 || cols == 0 || blowup == 0) return set_error(GL_E_ARG, "empty commit shape");
+    if (words_overflow(rows, blowup, cols)) return set_error(GL_E_ARG, "commit shape overflows the address space");
     if (!((P - 1) % rows == 0) || !((P - 1) % ext == 0))
         return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
     if (tree == TREE_ROWS && cols > kRowTreeMaxCols)

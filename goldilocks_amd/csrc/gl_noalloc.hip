@@ -57,6 +57,7 @@ extern "C" int gl_ntt_dev_noalloc(uint64_t *d, size_t n, size_t count, int inver
     if (!d) return set_error(GL_E_ARG, "null device pointer");
     if (plan_is_composite(*plan))
         return set_error(GL_E_UNSUPPORTED, "lengths with a factor 65537 need pool scratch; use gl_ntt_dev");
+    if (words_overflow(n, count)) return set_error(GL_E_ARG, "n * count words overflow the address space");
     const size_t words = plan_tmp_words(*plan, count, n);
     if (words > kScratchWords)
         return set_error(GL_E_ARG, "n * count exceeds the no-allocation scratch (2^24 words); use gl_ntt_dev");

@@ -161,6 +161,12 @@ int warm_plan(const Plan &plan, u64 S0, Keep *keep);
 // radix-65537 Rader pass), which grouped batch launches cannot include.
 bool plan_is_composite(const Plan &plan);
 size_t plan_tmp_words(const Plan &plan, u64 count, u64 bstride);
+// True when a * b * c words (8 bytes each) do not fit in size_t bytes.
+inline bool words_overflow(u64 a, u64 b, u64 c = 1) {
+    u64 ab, abc, bytes;
+    return __builtin_mul_overflow(a, b, &ab) || __builtin_mul_overflow(ab, c, &abc) ||
+           __builtin_mul_overflow(abc, (u64)8, &bytes);
+}
 // Device words of scratch a plan takes beyond tmp (the radix-65537 pass).
 size_t plan_extra_words(const Plan &plan, u64 count, u64 bstride);
 

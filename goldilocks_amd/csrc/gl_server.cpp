@@ -279,6 +279,7 @@ int gl_server_submit(gl_server *srv, const gl_job *job, uint64_t *ticket) {
             if (j->plan->passes.empty()) j->plan.reset();
         }
         if (s.count == 0 || (!s.data && s.n)) return set_error(GL_E_ARG, "server: empty or null NTT job");
+        if (words_overflow(s.n, s.count)) return set_error(GL_E_ARG, "server: n * count words overflow the address space");
         if (j->plan) {
             int rc = warm_plan(*j->plan, 1, &j->keep);
             if (rc) return rc;

@@ -104,6 +104,13 @@ def test_errors(gl):
         gl.ntt(np.zeros(0, dtype=np.uint64))
     with pytest.raises(ValueError):
         gl.strategy(3 * 7)
+    # n * count words beyond the address space: rejected before any launch
+    # (the pointer is never dereferenced)
+    import ctypes
+    import torch
+    x = torch.empty(4096, dtype=torch.int64, device="cuda")
+    rc = gl.lib().gl_ntt_dev(ctypes.c_void_p(x.data_ptr()), 4096, 1 << 62, 0, None)
+    assert rc == gl.GL_E_ARG and b"overflow" in gl.lib().gl_last_error()
 
 
 def test_device_columns(gl, oracle):

@@ -62,6 +62,9 @@ def test_noalloc_rejects(gl):
     # beyond the scratch: 2^25 words (pointer is never dereferenced)
     assert _noalloc(gl, x, 1 << 24, 2, 0) == 4  # GL_E_ARG
     assert b"2^24" in gl.lib().gl_last_error()
+    # n * count words beyond the address space
+    assert _noalloc(gl, x, 1 << 12, 1 << 62, 0) == 4
+    assert b"overflow" in gl.lib().gl_last_error()
     # a length with a factor 65537
     assert _noalloc(gl, x, 65537, 1, 0) == 5  # GL_E_UNSUPPORTED
     # a length whose tables were never built
