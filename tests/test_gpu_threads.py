@@ -4,8 +4,9 @@ plan cache allow, ntt/src/ntt/mod.rs:23-78).
 
 - two threads transforming different buffers on the same (NULL) stream get
   the same words as one thread alone (each call owns its scratch);
-- NTTs, Merkle trees and clear_cache issued concurrently from several
-  threads stay bit-exact (tables a call is using are never freed under it);
+- NTTs, Merkle trees, row-tree commits and clear_cache issued concurrently
+  from several threads stay bit-exact (tables a call is using are never
+  freed under it);
 - a captured batch plan keeps its twiddle tables across clear_cache;
 - removing an explicit schedule, or clearing the caches, puts the tune
   file's schedule back in force.
@@ -77,8 +78,9 @@ def test_two_threads_null_stream(gl, oracle):
 
 
 def test_concurrent_ntt_merkle_clear_cache(gl, oracle):
-    """Threads running NTTs (own streams and the NULL stream), Merkle trees
-    and clear_cache at the same time: every result is bit-exact."""
+    """Threads running NTTs (own streams and the NULL stream), Merkle trees,
+    row-tree commits and clear_cache at the same time: every result is
+    bit-exact."""
     import torch
     note("concurrent ntt / merkle / clear_cache")
     sizes = [1 << 16, 3 << 14, 257 << 6, 1 << 18]
@@ -113,13 +115,22 @@ def test_concurrent_ntt_merkle_clear_cache(gl, oracle):
         for _ in range(8):
             gl.clear_cache()
 
+    def commit_rows_worker():
+        for r in range(3):
+            row_results[r] = gl.commit_rows(mat, 2)
+
+    row_results = {}
+    mat = oracle.fill_random(256 * 12, 9).reshape(256, 12)
+    rows_expect = oracle.commit_rows(mat, 2)
     _run_threads([ntt_worker(0, True), ntt_worker(1, False), ntt_worker(2, True), ntt_worker(3, False),
-                  merkle_worker, clear_worker])
+                  merkle_worker, clear_worker, commit_rows_worker])
     torch.cuda.synchronize()
     for i in range(len(sizes)):
         assert np.array_equal(from_torch(xs[i]), expect[i]), f"n={sizes[i]}"
     for r, layers in results.items():
         assert layers == tree_expect, r
+    for r, (lde, layers) in row_results.items():
+        assert np.array_equal(lde, rows_expect[0]) and layers == rows_expect[1], r
 
 
 def test_batch_plan_survives_clear_cache(gl):
