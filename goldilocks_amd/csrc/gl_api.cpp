@@ -565,9 +565,23 @@ int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stre
     if (rc) return rc;
     if (n && (!d_a || !d_b_ntt)) return set_error(GL_E_ARG, "null device pointer");
     hipStream_t s = as_stream(stream);
-    if ((rc = ntt_dev_impl(d_a, n, 1, 1, false, s))) return rc;
-    launch_pointwise_mont(d_a, d_b_ntt, n, s);
-    GL_HIP(hipGetLastError());
+    PlanRef plan;
+    if ((rc = get_plan(n, &plan))) return rc;
+    const int last_kind = plan->passes.empty() ? (int)VK_COMPOSITE : plan->passes.back().v->kind;
+    if (last_kind == VK_TWO_STAGE || last_kind == VK_THREE_STAGE) {
+        // the forward transform's last pass multiplies output k by b[k] as
+        // it stores it (the pointwise step, fused: no extra pass over a)
+        const size_t words = plan_tmp_words(*plan, 1, n);
+        DevBuf tmp(words * 8, s);
+        if (words && !tmp.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
+        if ((rc = run_plan(*plan, d_a, tmp.as<u64>(), 1, 1, n, false, s, true, nullptr, nullptr, 0, d_b_ntt)))
+            return rc;
+    } else {
+        // n = 1, or a radix-257 / radix-65537 last pass: separate pointwise kernel
+        if ((rc = ntt_dev_impl(d_a, n, 1, 1, false, s))) return rc;
+        launch_pointwise_mont(d_a, d_b_ntt, n, s);
+        GL_HIP(hipGetLastError());
+    }
     return ntt_dev_impl(d_a, n, 1, 1, true, s);
 }
 

@@ -872,7 +872,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         a.coset = coset;
         a.coset_rows = coset_rows;
     }
-    if (inverse && last && row_scale) {
+    if (last && row_scale) {
         if (pp.v->kind != VK_TWO_STAGE && pp.v->kind != VK_THREE_STAGE)
             return set_error(GL_E_ARG, "row scale needs a single-kernel last pass");
         a.flags |= FLAG_ROW_SCALE;

@@ -150,7 +150,10 @@ struct PassCursor {
 // of length n (S0 = 1: contiguous transforms; S0 = cols: matrix columns).
 // tmp must hold count*bstride words when the plan has more than one pass.
 // input (optional): the first pass reads the transform from there and the
-// result lands in data (same layout), leaving input untouched.
+// result lands in data (same layout), leaving input untouched.  row_scale
+// (optional; last pass a single two- or three-stage kernel): output row r
+// is multiplied by row_scale[r] (Montgomery product), replacing the
+// inverse's 1/n.
 int run_plan(const Plan &plan, u64 *data, u64 *tmp, u64 S0, u64 count, u64 bstride, bool inverse,
              hipStream_t stream, bool scale_out = true, const u64 *input = nullptr, const u64 *coset = nullptr,
              u64 coset_rows = 0, const u64 *row_scale = nullptr);

@@ -446,3 +446,11 @@ int or_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
 
 int or_ntt(uint64_t *v, size_t n) { return or_ntt_batch(v, n, 1, 0); }
 int or_intt(uint64_t *v, size_t n) { return or_ntt_batch(v, n, 1, 1); }
+
+/* convolve.rs:13-20 (circular): a <- intt(ntt(a) * b_ntt), b already in NTT
+ * form, the product being the Field (Montgomery) product of the raw words. */
+int or_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
+    if (or_ntt(a, n)) return -1;
+    for (size_t i = 0; i < n; i++) a[i] = or_mont_mul(a[i], b_ntt[i]);
+    return or_intt(a, n);
+}

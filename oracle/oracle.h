@@ -64,6 +64,9 @@ int or_naive_intt(uint64_t *v, size_t n);           /* naive.rs:24-49 */
 int or_ntt(uint64_t *v, size_t n);                  /* mod.rs:98-102 */
 int or_intt(uint64_t *v, size_t n);                 /* mod.rs:104-120 */
 int or_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse); /* strategy(n).ntt */
+/* convolve.rs:13-20.  Pinned: the KAT [1,2,3]*[4,5,6] = [31,31,28]
+ * (convolve.rs:44-60) and the naive cyclic sum for small n. */
+int or_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n);
 /* X[k] of the forward transform, evaluated directly in O(n). */
 uint64_t or_eval_dft(const uint64_t *x, size_t n, uint64_t k);
 /* Host thread count used by or_ntt/or_ntt_batch outer loops, the commit's

@@ -43,6 +43,7 @@ def lib():
             "or_ntt": (i32, [vp, sz]), "or_intt": (i32, [vp, sz]),
             "or_ntt_batch": (i32, [vp, sz, sz, i32]),
             "or_eval_dft": (u64, [vp, sz, u64]),
+            "or_convolve": (i32, [vp, vp, sz]),
             "or_set_threads": (None, [i32]),
             "or_keccak_p1600": (None, [vp, i32]),
             "or_k12": (i32, [cp, sz, vp, sz]),
@@ -124,6 +125,16 @@ def ntt_batch(v, n, inverse=False):
 def eval_dft(x, k):
     x = _u64(x)
     return lib().or_eval_dft(_ptr(x), x.size, k)
+
+
+def convolve(a, b_ntt):
+    """circular(a, b) (convolve.rs:13-20) on copies: intt(ntt(a) * b_ntt)."""
+    a = _u64(a).copy()
+    b = _u64(b_ntt)
+    assert a.size == b.size
+    if lib().or_convolve(_ptr(a), _ptr(b), a.size) != 0:
+        raise ValueError("unsupported length")
+    return a
 
 
 def set_threads(t): lib().or_set_threads(int(t))

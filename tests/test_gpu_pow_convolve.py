@@ -1,8 +1,10 @@
 """Proof-of-work search and circular convolution (reference KATs)."""
+import os
+
 import numpy as np
 import pytest
 
-from gpu_util import note
+from gpu_util import from_torch, note, to_torch
 
 pytestmark = pytest.mark.gpu
 P = 0xFFFFFFFF00000001
@@ -66,3 +68,40 @@ def test_convolve_vs_naive(gl, oracle):
         bo = oracle.ntt(b.copy())
         prod = np.array([oracle.mont_mul(int(x), int(y)) for x, y in zip(ref, bo)], dtype=np.uint64)
         assert np.array_equal(got, oracle.intt(prod))
+
+
+def _last_kernel(gl, n):
+    info = gl.plan_info(n)
+    return info[-1]["kernel"] if info else "none"
+
+
+def test_convolve_fused_and_separate_pointwise(gl, oracle):
+    """The pointwise product rides on the forward transform's last pass when
+    that is one two- or three-stage kernel (2^12, 257*4 and 2^24: two-stage;
+    2^19, 2^20: three-stage), and is a separate kernel after a radix-257 or
+    radix-65537 last pass (257, 65537) or with no pass (n = 1): every result
+    equals the oracle's ntt, Montgomery product, intt; device and host entry
+    points agree."""
+    import torch
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        kinds = {}
+        for n in (1, 1 << 12, 257 * 4, 257, 65537, 1 << 19, 1 << 20, 1 << 24):
+            kinds[n] = _last_kernel(gl, n)
+            note(f"convolve n={n} last pass {kinds[n]}")
+            a = oracle.fill_random(n, 3 * n + 1)
+            b = oracle.fill_random(n, 3 * n + 2)
+            bo = oracle.ntt(b.copy())
+            want = oracle.convolve(a, bo)
+            da, db = to_torch(a), to_torch(bo)
+            gl.convolve(da, db)
+            torch.cuda.synchronize()
+            assert np.array_equal(from_torch(da), want), n
+            if n <= 1 << 16:
+                got = a.copy()
+                gl.convolve(got, bo)
+                assert np.array_equal(got, want), n
+        assert "pass3" in kinds[1 << 20] and "pass_kernel" in kinds[1 << 24] and "rader257" in kinds[257]
+    finally:
+        oracle.set_threads(1)
+

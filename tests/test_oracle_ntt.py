@@ -119,6 +119,20 @@ def test_convolution_kat(oracle):
     assert list(oracle.intt(prod)) == [31, 31, 28]
 
 
+def test_convolve(oracle):
+    """or_convolve (convolve.rs:13-20) on Montgomery words: the KAT
+    [1,2,3]*[4,5,6] = [31,31,28] (convolve.rs:44-60) and, for small n, the
+    cyclic sum of Field (Montgomery) products computed with big integers."""
+    mont = lambda v: np.array([(x << 64) % P for x in v], dtype=np.uint64)
+    assert list(oracle.convolve(mont([1, 2, 3]), oracle.ntt(mont([4, 5, 6])))) == list(mont([31, 31, 28]))
+    rinv = pow(2**64, -1, P)
+    for n in (1, 2, 3, 5, 16, 17, 60, 64, 257):
+        a = oracle.fill_random(n, 40 + n)
+        b = oracle.fill_random(n, 80 + n)
+        want = [sum(int(a[j]) * int(b[(i - j) % n]) for j in range(n)) * rinv % P for i in range(n)]
+        assert list(oracle.convolve(a, oracle.ntt(b))) == want, n
+
+
 @pytest.mark.slow
 def test_rader_65537(oracle):
     """rader.rs:125-129 skips this (naive takes minutes); sampled DFT instead."""
