@@ -63,8 +63,7 @@ int row_geometry(size_t leaves, RowGeometry &g) {
 int row_tree_check(size_t nrows, size_t cols) {
     if (nrows == 0 || cols == 0) return set_error(GL_E_ARG, "empty row tree");
     if (words_overflow(nrows, cols, 4)) return set_error(GL_E_ARG, "row tree shape overflows the address space");
-    if (cols > kRowTreeMaxCols)
-        return set_error(GL_E_SIZE, "row tree: rows of at most 1023 words (a single K12 node)");
+    if (cols > kRowTreeMaxCols) return set_error(GL_E_SIZE, "row tree: rows of at most 262143 words");
     return GL_OK;
 }
 
@@ -77,7 +76,11 @@ int row_tree_dev_impl(const u64 *d_rows, size_t nrows, size_t cols, uint8_t *d_o
     if (cap < need) return set_error(GL_E_ARG, "output buffer smaller than the row tree");
     if (!d_rows || !d_out) return set_error(GL_E_ARG, "null device pointer");
     if ((uintptr_t)d_rows % 8 || (uintptr_t)d_out % 8) return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
-    launch_row_tree(d_rows, nrows, (u32)cols, (u64 *)d_out, s);
+    // chaining values of K12's tree mode (rows over 1023 words), stream ordered
+    const u64 cvw = row_tree_cv_words(nrows, (u32)cols);
+    DevBuf cv(cvw * 8, s);
+    if (cvw && !cv.p) return set_error(GL_E_NOMEM, "row tree scratch allocation failed");
+    launch_row_tree(d_rows, nrows, (u32)cols, (u64 *)d_out, cv.as<u64>(), s);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -138,7 +141,7 @@ int commit_check(size_t rows, size_t cols, unsigned blowup, CommitTree tree) {
     if (!((P - 1) % rows == 0) || !((P - 1) % ext == 0))
         return set_error(GL_E_SIZE, "commit: rows and blowup*rows must divide p - 1");
     if (tree == TREE_ROWS && cols > kRowTreeMaxCols)
-        return set_error(GL_E_SIZE, "row-tree commit: rows of at most 1023 words (a single K12 node)");
+        return set_error(GL_E_SIZE, "row-tree commit: rows of at most 262143 words");
     return GL_OK;
 }
 
@@ -740,9 +743,15 @@ int gl_row_verify_dev(const uint64_t *d_rows, size_t cols, size_t leaves, const 
     if (k == 0) return GL_OK;
     if (!d_rows || !d_idx || !d_root32 || !d_ok || (g.depth && !d_paths))
         return set_error(GL_E_ARG, "null device pointer");
-    if ((uintptr_t)d_paths % 8 || (uintptr_t)d_root32 % 8) return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
-    launch_row_verify(g, d_rows, (u32)cols, d_idx, k, (const u64 *)d_paths, (const u64 *)d_root32, (int *)d_ok,
-                      as_stream(stream));
+    if ((uintptr_t)d_rows % 8 || (uintptr_t)d_paths % 8 || (uintptr_t)d_root32 % 8)
+        return set_error(GL_E_ARG, "row tree buffers must be 8-byte aligned");
+    hipStream_t s = as_stream(stream);
+    // the opened rows' leaf hashes, then the walks up to the root
+    const u64 cvw = row_tree_cv_words(k, (u32)cols);
+    DevBuf hashes(k * 32, s), cv(cvw * 8, s);
+    if (!hashes.p || (cvw && !cv.p)) return set_error(GL_E_NOMEM, "row verify scratch allocation failed");
+    launch_row_leaves(d_rows, k, (u32)cols, hashes.as<u64>(), cv.as<u64>(), s);
+    launch_row_verify(g, hashes.as<u64>(), d_idx, k, (const u64 *)d_paths, (const u64 *)d_root32, (int *)d_ok, s);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }

@@ -146,21 +146,25 @@ struct MerkleGeometry {
 void launch_merkle_open(const MerkleGeometry &g, const u64 *blocks, u64 k, uint8_t *out, hipStream_t s);
 void launch_merkle_verify(const MerkleGeometry &g, const uint8_t *paths, const u64 *blocks, u64 k,
                           const uint8_t *root32, int *ok, hipStream_t s);
-// Row tree (the binary tree of row hashes): leaves = K12 of each row of
-// cols <= kRowTreeMaxCols words (K12's single node), then the layers of
-// parents up to the root, all written to out (4 words a node).  Layer l has
-// n[l] nodes starting at node off[l] of that output; depth = hashed layers
-// above the leaves.
-constexpr unsigned kRowTreeMaxCols = 1023;
+// Row tree (the binary tree of row hashes): leaves = K12 of each row (K12's
+// single node up to kSingleNodeCols words, its tree mode up to
+// kRowTreeMaxCols, whose chaining values need row_tree_cv_words scratch
+// words at cv), then the layers of parents up to the root, all written to
+// out (4 words a node).  Layer l has n[l] nodes starting at node off[l] of
+// that output; depth = hashed layers above the leaves.
+constexpr unsigned kSingleNodeCols = 1023;
+constexpr unsigned kRowTreeMaxCols = 256 * 1024 - 1;  // at most 255 chunk chaining values
 constexpr int kMaxRowDepth = 64;
 struct RowGeometry {
     u64 off[kMaxRowDepth + 1];
     u64 n[kMaxRowDepth + 1];
     int depth;
 };
-void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, hipStream_t s);
+u64 row_tree_cv_words(u64 nrows, u32 cols);
+void launch_row_leaves(const u64 *rows, u64 nrows, u32 cols, u64 *out, u64 *cv, hipStream_t s);
+void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, u64 *cv, hipStream_t s);
 void launch_row_open(const RowGeometry &g, const u64 *layers, const u64 *idx, u64 k, u64 *out, hipStream_t s);
-void launch_row_verify(const RowGeometry &g, const u64 *rows, u32 cols, const u64 *idx, u64 k, const u64 *paths,
+void launch_row_verify(const RowGeometry &g, const u64 *leaves, const u64 *idx, u64 k, const u64 *paths,
                        const u64 *root, int *ok, hipStream_t s);
 void launch_pow_search(const uint8_t *challenge32, u64 threshold, u64 start, u64 count,
                        unsigned long long *best, hipStream_t s);

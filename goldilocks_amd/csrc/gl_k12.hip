@@ -253,10 +253,11 @@ bool launch_k12_layer(const uint8_t *in, u64 len, uint8_t *out, u64 *stage_in, b
 // Leaf r = K12(row r): the row's 8 * cols bytes and the empty customization's
 // length byte 0x00, absorbed 21 words (168 bytes) at a time by Keccak-p[1600,
 // 12] with suffix 0x07 and 0x80 at byte 167 of the last block (TurboSHAKE128,
-// D = 0x07: K12's single node, rows up to kRowTreeMaxCols words).  The word
-// after the data holds the bytes 0x00, 0x07: the constant 0x0700.  A parent
-// is K12(left || right), 64 bytes in one permutation; the last node of an
-// odd layer moves up unchanged.
+// D = 0x07: K12's single node, rows up to kSingleNodeCols words).  The word
+// after the data holds the bytes 0x00, 0x07: the constant 0x0700.  Wider
+// rows use K12's tree mode (k12_rows_cv_kernel, k12_rows_node_kernel).  A
+// parent is K12(left || right), 64 bytes in one permutation; the last node
+// of an odd layer moves up unchanged.
 
 // Absorb and permute row w (cols words) into a[0..3] = its 32-byte hash.
 __device__ __forceinline__ void row_leaf(const u64 *w, u32 cols, u64 *a) {
@@ -301,6 +302,74 @@ __global__ __launch_bounds__(256) void k12_rows_kernel(const u64 *__restrict__ r
     for (u64 r = blockIdx.x * (u64)blockDim.x + threadIdx.x; r < nrows; r += (u64)gridDim.x * blockDim.x) {
         u64 a[25];
         row_leaf(rows + r * cols, cols, a);
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[4 * r + i] = a[i];
+    }
+}
+
+// Tree-mode leaves (RFC 9861 section 3.2), rows of kSingleNodeCols + 1 words
+// or more: S = row bytes || 0x00 is cut into kChunkWords-word (8192-byte)
+// chunks S_0 .. S_{n-1} (n = cols / kChunkWords + 1); chunk i >= 1 gives the
+// chaining value CV_i = TurboSHAKE128(S_i, D = 0x0B, 32 bytes), and the leaf
+// is TurboSHAKE128(S_0 || 0x03 0x00^7 || CV_1 .. CV_{n-1} ||
+// length_encode(n - 1) || 0xFF 0xFF, D = 0x06).  Every piece is whole words
+// (the rows are words), so each message is produced word by word.
+constexpr u32 kChunkWords = 1024;
+
+// TurboSHAKE128 of the nw words word(0 .. nw-1), the last of which already
+// holds the domain byte; 0x80 goes to byte 167 of the last block.
+template <class Word>
+__device__ __forceinline__ void absorb_words(Word word, u32 nw, u64 *a) {
+#pragma unroll
+    for (int i = 0; i < 25; i++) a[i] = 0;
+    const u32 nb = (nw + 20) / 21;
+    for (u32 b = 0; b < nb; b++) {
+        const u32 base = 21 * b;
+#pragma unroll
+        for (int i = 0; i < 21; i++)
+            if (base + i < nw) a[i] ^= word(base + i);
+        if (b + 1 == nb) a[20] ^= 0x8000000000000000ull;
+        keccak_p12(a);
+    }
+}
+
+// One thread per (row, chunk c >= 1): cv[(row * (nch - 1) + c - 1) * 4 ..].
+// Chunk c holds row words [c * kChunkWords, min((c + 1) * kChunkWords, cols))
+// and, if it is the last, the byte 0x00 of S: its domain word is 0x0B00
+// then, else 0x0B (the chunk ends on a word boundary).
+__global__ __launch_bounds__(256) void k12_rows_cv_kernel(const u64 *__restrict__ rows, u64 nrows, u32 cols,
+                                                          u32 nch, u64 *__restrict__ cv) {
+    const u64 per = nch - 1, total = nrows * per;
+    for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
+        const u64 r = t / per;
+        const u32 c = (u32)(t - r * per) + 1;
+        const u64 *src = rows + r * cols + (u64)kChunkWords * c;
+        const bool last = c + 1 == nch;
+        const u32 nd = last ? cols - kChunkWords * c : kChunkWords;
+        const u64 dom = last ? 0x0B00ull : 0x0Bull;
+        u64 a[25];
+        absorb_words([&](u32 j) { return j < nd ? src[j] : dom; }, nd + 1, a);
+#pragma unroll
+        for (int i = 0; i < 4; i++) cv[4 * t + i] = a[i];
+    }
+}
+
+// One thread per row: the final node S_0 || 0x03 0x00^7 || CVs || tail,
+// tail = length_encode(n - 1) (n - 1 < 256: two bytes), 0xFF 0xFF and the
+// domain byte 0x06 in one word.
+__global__ __launch_bounds__(256) void k12_rows_node_kernel(const u64 *__restrict__ rows, u64 nrows, u32 cols,
+                                                            u32 nch, const u64 *__restrict__ cv,
+                                                            u64 *__restrict__ out) {
+    const u32 ncv = 4 * (nch - 1);
+    const u64 tail = (u64)(nch - 1) | (0x01ull << 8) | (0xFFull << 16) | (0xFFull << 24) | (0x06ull << 32);
+    for (u64 r = blockIdx.x * (u64)blockDim.x + threadIdx.x; r < nrows; r += (u64)gridDim.x * blockDim.x) {
+        const u64 *w = rows + r * cols, *c = cv + r * ncv;
+        u64 a[25];
+        absorb_words(
+            [&](u32 j) {
+                return j < kChunkWords ? w[j] : j == kChunkWords ? 0x03ull : j <= kChunkWords + ncv ? c[j - kChunkWords - 1] : tail;
+            },
+            kChunkWords + 2 + ncv, a);
 #pragma unroll
         for (int i = 0; i < 4; i++) out[4 * r + i] = a[i];
     }
@@ -357,10 +426,31 @@ __global__ __launch_bounds__(256) void k12_pairs_tail_kernel(const u64 *__restri
     }
 }
 
-void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, hipStream_t s) {
+static unsigned grid_for(u64 threads) {
+    u64 g = (threads + 255) / 256;
+    return (unsigned)(g < 65536 ? g : 65536);
+}
+
+u64 row_tree_cv_words(u64 nrows, u32 cols) {
+    return cols > kSingleNodeCols ? nrows * (u64)(cols / kChunkWords) * 4 : 0;
+}
+
+void launch_row_leaves(const u64 *rows, u64 nrows, u32 cols, u64 *out, u64 *cv, hipStream_t s) {
     if (nrows == 0) return;
-    auto grid_for = [](u64 threads) { u64 g = (threads + 255) / 256; return (unsigned)(g < 65536 ? g : 65536); };
-    hipLaunchKernelGGL(k12_rows_kernel, dim3(grid_for(nrows)), dim3(256), 0, s, rows, nrows, cols, out);
+    if (cols <= kSingleNodeCols) {
+        hipLaunchKernelGGL(k12_rows_kernel, dim3(grid_for(nrows)), dim3(256), 0, s, rows, nrows, cols, out);
+        return;
+    }
+    const u32 nch = cols / kChunkWords + 1;
+    hipLaunchKernelGGL(k12_rows_cv_kernel, dim3(grid_for(nrows * (nch - 1))), dim3(256), 0, s, rows, nrows, cols, nch,
+                       cv);
+    hipLaunchKernelGGL(k12_rows_node_kernel, dim3(grid_for(nrows)), dim3(256), 0, s, rows, nrows, cols, nch,
+                       (const u64 *)cv, out);
+}
+
+void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, u64 *cv, hipStream_t s) {
+    if (nrows == 0) return;
+    launch_row_leaves(rows, nrows, cols, out, cv, s);
     u64 n = nrows;
     u64 *layer = out;
     while (n > kRowTail) {
@@ -388,9 +478,9 @@ __global__ void row_open_kernel(RowGeometry g, const u64 *__restrict__ layers, c
     }
 }
 
-// ok[i] = 1 when row rows[i] (cols words) at leaf idx[i] with opening
-// paths[i] hashes up to root.
-__global__ __launch_bounds__(64) void row_verify_kernel(RowGeometry g, const u64 *__restrict__ rows, u32 cols,
+// ok[i] = 1 when leaf hash leaves[i] (of the opened row) at leaf idx[i] with
+// opening paths[i] hashes up to root.
+__global__ __launch_bounds__(64) void row_verify_kernel(RowGeometry g, const u64 *__restrict__ leaves,
                                                         const u64 *__restrict__ idx, u64 k,
                                                         const u64 *__restrict__ paths,
                                                         const u64 *__restrict__ root, int *__restrict__ ok) {
@@ -400,9 +490,7 @@ __global__ __launch_bounds__(64) void row_verify_kernel(RowGeometry g, const u64
             ok[i] = 0;
             continue;
         }
-        u64 a[25];
-        row_leaf(rows + i * cols, cols, a);
-        u64 h[4] = {a[0], a[1], a[2], a[3]};
+        u64 h[4] = {leaves[4 * i], leaves[4 * i + 1], leaves[4 * i + 2], leaves[4 * i + 3]};
         for (int l = 0; l < g.depth; l++) {
             const u64 *sib = paths + 4 * (i * (u64)g.depth + l);
             if ((node ^ 1) < g.n[l]) {
@@ -423,11 +511,11 @@ void launch_row_open(const RowGeometry &g, const u64 *layers, const u64 *idx, u6
                        k, out);
 }
 
-void launch_row_verify(const RowGeometry &g, const u64 *rows, u32 cols, const u64 *idx, u64 k, const u64 *paths,
+void launch_row_verify(const RowGeometry &g, const u64 *leaves, const u64 *idx, u64 k, const u64 *paths,
                        const u64 *root, int *ok, hipStream_t s) {
     if (k == 0) return;
     u64 grid = (k + 63) / 64;
-    hipLaunchKernelGGL(row_verify_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(64), 0, s, g, rows, cols,
+    hipLaunchKernelGGL(row_verify_kernel, dim3((unsigned)(grid < 65536 ? grid : 65536)), dim3(64), 0, s, g, leaves,
                        idx, k, paths, root, ok);
 }
 

@@ -72,6 +72,7 @@ CommitTree tree_of(const gl_job &j) { return j.kind == GL_JOB_COMMIT_ROWS ? TREE
 size_t commit_footprint(const gl_job &j) {
     size_t rows = j.n, cols = j.count, ext = rows * j.blowup;
     size_t f = rows * cols * 8 + ext * cols * 8;  // coef + tmp
+    if (j.kind == GL_JOB_COMMIT_ROWS) f += row_tree_cv_words(ext, (u32)cols) * 8;  // K12 tree-mode chaining values
     if (!j.device) f += rows * cols * 8 + ext * cols * 8 + commit_tree_size(rows, cols, j.blowup, tree_of(j));
     return f;
 }

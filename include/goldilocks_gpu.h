@@ -246,11 +246,12 @@ int gl_commit_dev(const uint64_t *d_m, size_t rows, size_t cols, unsigned blowup
 /* ---- Row tree and row-tree commit (no reference function: the binary tree
  * of row hashes, an alternative to merkle_tree over the commit's bytes) -----
  * Leaf r = KangarooTwelve of row r's 8*cols little-endian bytes (empty
- * customization, 32-byte output; rows of at most 1023 words, K12's single
- * node, GL_E_SIZE beyond): TurboSHAKE128 of the row and 0x00 with D = 0x07,
- * several Keccak-p[1600,12] blocks per row (the same permutation, padding
- * and constants as k12::<32>, pcs/src/k12/generic.rs:62-84, which is this
- * for inputs up to 166 bytes).  A layer of n nodes gives ceil(n/2) parents,
+ * customization, 32-byte output, RFC 9861): up to 1023 words K12's single
+ * node, TurboSHAKE128 of the row and 0x00 with D = 0x07 over several
+ * Keccak-p[1600,12] blocks (the same permutation, padding and constants as
+ * k12::<32>, pcs/src/k12/generic.rs:62-84, which is this for inputs up to
+ * 166 bytes); from 1024 words K12's tree mode over 8192-byte chunks; rows
+ * of at most 262143 words (GL_E_SIZE beyond).  A layer of n nodes gives ceil(n/2) parents,
  * parent j = k12::<32>(node 2j || node 2j+1); the last node of an odd layer
  * moves up unchanged; layers until one node, the root.  Layers output: every
  * layer in order, leaves first, 32 bytes per node, gl_row_tree_size(leaves)

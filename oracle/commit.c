@@ -60,7 +60,7 @@ int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
 
 int or_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
                    uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len) {
-    if (cols > 1023 || lde(m, rows, cols, blowup, lde_out)) return -1;
+    if (lde(m, rows, cols, blowup, lde_out)) return -1;
     return or_row_tree(lde_out, rows * (size_t)blowup, cols, layers_out, cap, out_len);
 }
 

@@ -11,6 +11,7 @@
  */
 #include "oracle.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 /* FIPS 202 round constants; Keccak-p[1600,12] uses the last 12
@@ -125,16 +126,60 @@ void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]) {
     or_keccak_sponge(in, len, 136, 0x06, 24, out, 32);
 }
 
-/* KangarooTwelve with the empty customization string for messages that fit
- * its single node, len + 1 <= 8192 bytes: TurboSHAKE128(M || 0x00, D = 0x07)
- * (RFC 9861 sections 2-3).  For len <= 166 this is or_k12 (generic.rs:62-84,
- * one permutation); longer messages absorb several 168-byte blocks. */
+/* length_encode (RFC 9861 section 3.3): x as big-endian bytes without
+ * leading zeros, then the count of those bytes (0 -> the single byte 0x00). */
+static size_t length_encode(size_t x, uint8_t out[9]) {
+    uint8_t be[8];
+    size_t n = 0;
+    for (; x; x >>= 8) be[n++] = (uint8_t)x;
+    for (size_t i = 0; i < n; i++) out[i] = be[n - 1 - i];
+    out[n] = (uint8_t)n;
+    return n + 1;
+}
+
+/* KangarooTwelve with the empty customization string (RFC 9861 section 3):
+ * S = M || length_encode(0) = M || 0x00.  Up to 8192 bytes of S (len <=
+ * 8191) it is TurboSHAKE128(S, D = 0x07), for len <= 166 the single
+ * permutation of or_k12 (generic.rs:62-84); longer S is cut into 8192-byte
+ * chunks, chunk i >= 1 gives CV_i = TurboSHAKE128(S_i, D = 0x0B, 32 bytes)
+ * and the result is TurboSHAKE128(S_0 || 0x03 0x00^7 || CV_1 .. CV_{n-1} ||
+ * length_encode(n-1) || 0xFF 0xFF, D = 0x06). */
 int or_k12_long(const uint8_t *in, size_t len, uint8_t *out, size_t outlen) {
-    if (len > 8191 || outlen > 168) return -1;
-    uint8_t msg[8192];
-    memcpy(msg, in, len);
-    msg[len] = 0x00; /* length_encode(0) of the customization string */
-    return or_keccak_sponge(msg, len + 1, 168, 0x07, 12, out, outlen);
+    if (outlen > 168) return -1;
+    const size_t slen = len + 1, chunk = 8192;
+    uint8_t *S = (uint8_t *)malloc(slen);
+    if (!S) return -1;
+    if (len) memcpy(S, in, len);
+    S[len] = 0x00; /* length_encode(0) of the customization string */
+    int rc;
+    if (slen <= chunk) {
+        rc = or_keccak_sponge(S, slen, 168, 0x07, 12, out, outlen);
+    } else {
+        const size_t nch = (slen + chunk - 1) / chunk;
+        uint8_t enc[9];
+        const size_t el = length_encode(nch - 1, enc);
+        const size_t nlen = chunk + 8 + 32 * (nch - 1) + el + 2;
+        uint8_t *node = (uint8_t *)malloc(nlen);
+        if (!node) {
+            free(S);
+            return -1;
+        }
+        memcpy(node, S, chunk);
+        node[chunk] = 0x03;
+        memset(node + chunk + 1, 0, 7);
+        rc = 0;
+        for (size_t i = 1; i < nch && !rc; i++) {
+            const size_t off = i * chunk, l = slen - off < chunk ? slen - off : chunk;
+            rc = or_keccak_sponge(S + off, l, 168, 0x0B, 12, node + chunk + 8 + 32 * (i - 1), 32);
+        }
+        memcpy(node + chunk + 8 + 32 * (nch - 1), enc, el);
+        node[nlen - 2] = 0xFF;
+        node[nlen - 1] = 0xFF;
+        if (!rc) rc = or_keccak_sponge(node, nlen, 168, 0x06, 12, out, outlen);
+        free(node);
+    }
+    free(S);
+    return rc;
 }
 
 #define BLOCK_SIZE 160
@@ -193,8 +238,9 @@ int or_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
 }
 
 /* ---- Row tree: the binary tree of row hashes (no reference function).
- * Leaf r = K12(row r), the row's 8 * cols little-endian bytes (cols <= 1023,
- * K12's single node); a layer of n nodes gives ceil(n / 2) nodes, node j =
+ * Leaf r = K12(row r), the row's 8 * cols little-endian bytes (K12's single
+ * node up to 1023 words, its tree mode above); a layer of n nodes gives
+ * ceil(n / 2) nodes, node j =
  * K12(node 2j || node 2j + 1) (64 bytes, one permutation), the last node of
  * an odd layer moving up unchanged; layers until one node, the root.  The
  * output is every layer in order, leaves first, 32 bytes per node. */
@@ -236,7 +282,7 @@ static void pair_range(void *arg, size_t i) {
 int or_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *out, size_t cap, size_t *out_len) {
     const size_t need = or_row_tree_size(nrows);
     if (out_len) *out_len = need;
-    if (nrows == 0 || cols == 0 || cols > 1023 || cap < need) return -1;
+    if (nrows == 0 || cols == 0 || cap < need) return -1;
     /* Field words are little-endian u64 on this host: the byte view is the memory */
     RowCtx cx = {(const uint8_t *)rows, nrows, 8 * cols, out};
     or_par_for((nrows + 63) / 64, 64 * (cols / 21 + 1) * 1024, leaf_range, &cx);

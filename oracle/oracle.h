@@ -86,8 +86,10 @@ void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]); /* pinning aid
  * outlen <= rate.  Pinned: SHAKE128 / SHA3-256 parameters vs hashlib. */
 int or_keccak_sponge(const uint8_t *in, size_t len, size_t rate, uint8_t suffix, int rounds,
                      uint8_t *out, size_t outlen);
-/* Single-node K12 (len <= 8191): equals or_k12 up to 166 bytes.  Pinned:
- * or_k12 for len <= 166 and the XKCP KATs for 17^2 and 17^3 pattern bytes. */
+/* KangarooTwelve, empty customization, any length (single node up to 8191
+ * bytes, tree mode above); equals or_k12 up to 166 bytes.  Pinned: or_k12
+ * for len <= 166 and the XKCP / RFC 9861 vectors for 17^2, 17^3, 8191,
+ * 8192, 17^4 and 17^5 pattern bytes. */
 int or_k12_long(const uint8_t *in, size_t len, uint8_t *out, size_t outlen);
 size_t or_next_layer_size(size_t len);              /* mod.rs:33-35 */
 void or_process_layer(const uint8_t *in, size_t len, uint8_t *out); /* generic.rs:86-94 */
@@ -98,7 +100,7 @@ size_t or_merkle_tree_size(size_t len);
 /* Row tree (no reference function; the optional binary tree of row hashes):
  * leaf r = or_k12_long(row r bytes), node = or_k12(left || right), the last
  * node of an odd layer moves up unchanged; every layer, leaves first.
- * Returns -1 for cols > 1023, empty shapes or cap too small.  Pinned by a
+ * Returns -1 for empty shapes or cap too small.  Pinned by a
  * Python assembly of the layers over or_k12_long / or_k12. */
 int or_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *out, size_t cap, size_t *out_len);
 size_t or_row_tree_size(size_t leaves);

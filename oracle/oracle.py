@@ -169,10 +169,10 @@ def keccak_sponge(data: bytes, rate, suffix, rounds, outlen) -> bytes:
 
 
 def k12_long(data: bytes, outlen=32) -> bytes:
-    """Single-node KangarooTwelve, messages up to 8191 bytes."""
+    """KangarooTwelve (empty customization) of any length."""
     out = ctypes.create_string_buffer(outlen)
     if lib().or_k12_long(bytes(data), len(data), out, outlen) != 0:
-        raise ValueError("k12_long input longer than 8191 bytes or output longer than 168")
+        raise ValueError("k12_long output longer than 168 bytes")
     return out.raw
 
 

@@ -19,10 +19,13 @@ def gl():
 # leaf counts cover one leaf, odd layers at every height, the single-workgroup
 # tail alone (<= 512 nodes) and the layer kernel above it; column counts cover
 # one block, a block boundary (21 words = 168 bytes: the padding word opens a
-# block of its own), one word either side of it, and the 1023-word maximum
+# block of its own), one word either side of it, the 1023-word single-node
+# maximum and K12's tree mode beyond it (1024 words: a last chunk holding
+# only the customization byte; 1025, 2048, 3000 and 9000 words: one to eight
+# chained chunks)
 @pytest.mark.parametrize("nrows,cols", [(1, 1), (1, 64), (2, 3), (3, 20), (5, 21), (6, 22), (7, 41), (9, 42),
                                         (33, 64), (512, 8), (513, 8), (1000, 64), (4097, 2), (100, 1023),
-                                        (2**16 + 3, 5)])
+                                        (2**16 + 3, 5), (3, 1024), (5, 1025), (600, 2048), (2, 3000), (3, 9000)])
 def test_row_tree_vs_oracle(gl, oracle, nrows, cols):
     note(f"row tree {nrows}x{cols}")
     m = oracle.fill_random(nrows * cols, 7 * nrows + cols).reshape(nrows, cols)
@@ -34,16 +37,24 @@ def test_row_tree_vs_oracle(gl, oracle, nrows, cols):
 
 
 def test_row_tree_errors(gl):
-    with pytest.raises(ValueError):
-        gl.row_tree(np.zeros((4, 1024), dtype=np.uint64))
+    import torch
+    x = torch.empty(8, dtype=torch.int64, device="cuda")
+    # rows wider than 262143 words (pointer never dereferenced)
+    rc = gl.lib().gl_row_tree_dev(_p(x), 2, 262144, _p(x), 1 << 20, None, None)
+    assert rc == gl.GL_E_SIZE and b"262143" in gl.lib().gl_last_error()
     with pytest.raises(ValueError):
         gl.row_tree(np.zeros((0, 4), dtype=np.uint64))
     with pytest.raises(ValueError):
-        gl.commit_rows(np.zeros((4, 1024), dtype=np.uint64), 2)
+        gl.commit_rows(np.zeros((7, 4), dtype=np.uint64), 2)
+
+
+def _p(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
 
 
 @pytest.mark.parametrize("rows,cols,blowup", [(8, 3, 2), (16, 64, 4), (12, 21, 2), (1, 5, 1), (1024, 64, 2),
-                                              (3 * 2**10, 5, 2), (257 * 4, 3, 2)])
+                                              (3 * 2**10, 5, 2), (257 * 4, 3, 2), (64, 1500, 2)])
 def test_commit_rows_vs_oracle(gl, oracle, rows, cols, blowup):
     note(f"commit_rows {rows}x{cols} blowup {blowup}")
     m = oracle.fill_random(rows * cols, rows * 17 + cols).reshape(rows, cols)
@@ -56,10 +67,9 @@ def test_commit_rows_vs_oracle(gl, oracle, rows, cols, blowup):
     assert dt.cpu().numpy().tobytes() == b"".join(layers_o)
 
 
-@pytest.mark.parametrize("leaves", [1, 2, 5, 513, 1000, 4099])
-def test_row_open_verify(gl, oracle, leaves):
+@pytest.mark.parametrize("leaves,cols", [(1, 6), (2, 6), (5, 6), (513, 6), (1000, 6), (4099, 6), (77, 2100)])
+def test_row_open_verify(gl, oracle, leaves, cols):
     import torch
-    cols = 6
     note(f"row open/verify {leaves} leaves")
     m = oracle.fill_random(leaves * cols, 3 + leaves).reshape(leaves, cols)
     layers_o = oracle.row_tree(m)

@@ -35,11 +35,11 @@ def test_commit_rejects(oracle):
 def test_commit_rows(oracle):
     """The row-tree commit: the same extension as commit, the row tree over it."""
     import pytest
-    for rows, cols, blowup in [(8, 3, 2), (16, 64, 4), (12, 21, 2), (1, 5, 1)]:
+    for rows, cols, blowup in [(8, 3, 2), (16, 64, 4), (12, 21, 2), (1, 5, 1), (4, 1024, 2)]:
         m = oracle.fill_random(rows * cols, 91 + rows).reshape(rows, cols)
         lde, layers = oracle.commit_rows(m, blowup)
         lde5, _ = oracle.commit(m, blowup)
         assert np.array_equal(lde, lde5)
         assert layers == oracle.row_tree(lde)
     with pytest.raises(ValueError):
-        oracle.commit_rows(np.zeros((4, 1024), dtype=np.uint64), 2)
+        oracle.commit_rows(np.zeros((7, 4), dtype=np.uint64), 2)

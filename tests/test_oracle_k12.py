@@ -115,9 +115,18 @@ def test_k12_long(oracle):
         "0c 31 5e bc de db f6 14 26 de 7d cf 8f b7 25 d1 e7 46 75 d7 f5 32 7a 50 67 f3 67 b1 08 ec b6 7c")
     assert oracle.k12_long(ptn(17 ** 3), 32) == hx(
         "cb 55 2e 2e c7 7d 99 10 70 1d 57 8b 45 7d df 77 2c 12 e3 22 e4 ee 7f e4 17 f9 2c 75 8f 0d 59 d0")
-    oracle.k12_long(bytes(8191), 32)
+    # the last single-node length and the first tree-mode lengths (RFC 9861
+    # section 5): 8191 and 8192 bytes, then 17^4 (11 chunks) and 17^5 (174)
+    assert oracle.k12_long(ptn(8191), 32) == hx(
+        "1b 57 76 36 f7 23 64 3e 99 0c c7 d6 a6 59 83 74 36 fd 6a 10 36 26 60 0e b8 30 1c d1 db e5 53 d6")
+    assert oracle.k12_long(ptn(8192), 32) == hx(
+        "48 f2 56 f6 77 2f 9e df b6 a8 b6 61 ec 92 dc 93 b9 5e bd 05 a0 8a 17 b3 9a e3 49 08 70 c9 26 c3")
+    assert oracle.k12_long(ptn(17 ** 4), 32) == hx(
+        "87 01 04 5e 22 20 53 45 ff 4d da 05 55 5c bb 5c 3a f1 a7 71 c2 b8 9b ae f3 7d b4 3d 99 98 b9 fe")
+    assert oracle.k12_long(ptn(17 ** 5), 32) == hx(
+        "84 4d 61 09 33 b1 b9 96 3c bd eb 5a e3 b6 b0 5c c7 cb d6 7c ee df 88 3e b6 78 a0 a8 e0 37 16 82")
     with pytest.raises(ValueError):
-        oracle.k12_long(bytes(8192), 32)
+        oracle.k12_long(b"", 169)
 
 
 def row_tree_py(oracle, rows):
@@ -134,7 +143,7 @@ def row_tree_py(oracle, rows):
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 64), (2, 3), (3, 20), (5, 21), (6, 22), (7, 41), (9, 42),
-                                   (33, 64), (100, 1023), (257, 5)])
+                                   (33, 64), (100, 1023), (257, 5), (3, 1024), (2, 1025), (2, 2048), (1, 3000)])
 def test_row_tree(oracle, shape):
     import numpy as np
     rows = oracle.fill_random(shape[0] * shape[1], 11 + shape[0]).reshape(shape)
@@ -143,4 +152,4 @@ def test_row_tree(oracle, shape):
     assert sum(len(l) for l in got) == oracle.row_tree_size(shape[0])
     assert len(got[-1]) == 32
     with pytest.raises(ValueError):
-        oracle.row_tree(np.zeros((4, 1024), dtype=np.uint64))
+        oracle.row_tree(np.zeros((0, 4), dtype=np.uint64))
