@@ -465,7 +465,7 @@ void launch_row_tree(const u64 *rows, u64 nrows, u32 cols, u64 *out, u64 *cv, hi
 // Opening i of leaf idx[i]: for layer l < depth, the sibling of node
 // idx[i] >> l (4 words), or zeros where that node has none (it moves up
 // unchanged) or idx[i] is not a leaf.  One thread per (opening, layer).
-__global__ void row_open_kernel(RowGeometry g, const u64 *__restrict__ layers, const u64 *__restrict__ idx, u64 k,
+__global__ __launch_bounds__(256) void row_open_kernel(RowGeometry g, const u64 *__restrict__ layers, const u64 *__restrict__ idx, u64 k,
                                 u64 *__restrict__ out) {
     const u64 total = k * (u64)g.depth;
     for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void pow_kernel(const u64 *__restrict__ ch, u6
 }
 
 // Single K12 hash with up to 168 output bytes (k12::<H>, generic.rs:62-84).
-__global__ void k12_single_kernel(const uint8_t *in, u64 len, uint8_t *out, u64 outlen) {
+__global__ __launch_bounds__(64) void k12_single_kernel(const uint8_t *in, u64 len, uint8_t *out, u64 outlen) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     u64 a[25];
     absorb_padded(a, in, len);
@@ -564,7 +564,7 @@ void launch_k12_single(const uint8_t *in, u64 len, uint8_t *out, u64 outlen, hip
 // sits in block b_{l+1} at byte 32 * (b_l % 5).
 
 // out[(i * depth + l) * 160 + j], one thread per byte.
-__global__ void merkle_open_kernel(MerkleGeometry g, const u64 *__restrict__ blocks, u64 k, uint8_t *__restrict__ out) {
+__global__ __launch_bounds__(256) void merkle_open_kernel(MerkleGeometry g, const u64 *__restrict__ blocks, u64 k, uint8_t *__restrict__ out) {
     const u64 total = k * (u64)g.depth * 160;
     for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
         const u64 j = t % 160, il = t / 160, l = il % g.depth, i = il / g.depth;
@@ -576,7 +576,7 @@ __global__ void merkle_open_kernel(MerkleGeometry g, const u64 *__restrict__ blo
 }
 
 // ok[i] = 1 when opening i of leaf block blocks[i] hashes up to root32.
-__global__ void merkle_verify_kernel(MerkleGeometry g, const uint8_t *__restrict__ paths, const u64 *__restrict__ blocks,
+__global__ __launch_bounds__(64) void merkle_verify_kernel(MerkleGeometry g, const uint8_t *__restrict__ paths, const u64 *__restrict__ blocks,
                                      u64 k, const uint8_t *__restrict__ root32, int *__restrict__ ok) {
     for (u64 i = blockIdx.x * (u64)blockDim.x + threadIdx.x; i < k; i += (u64)gridDim.x * blockDim.x) {
         u64 b = blocks[i];
