@@ -55,7 +55,7 @@ int main() {
         uint64_t s = 0x9e3779b97f4a7c15ull * (t + 1);
         for (int it = 0; it < 3000; it++) {
             const uint64_t n = sizes[rng(s) % sizes.size()];
-            const int op = (int)(rng(s) % 16);
+            const int op = (int)(rng(s) % 17);
             if (op < 8) {
                 uint32_t a[8], b[8], c[8];
                 size_t np = 0;
@@ -89,6 +89,17 @@ int main() {
                 gl_clear_cache();
             } else if (op == 14) {
                 CHECK(gl_plan_set(n * 7, nullptr, 0) == GL_E_SIZE);
+            } else if (op == 15) {
+                // row tree sizes: leaves + ceil halves up to the root
+                const size_t leaves = 1 + rng(s) % 5000;
+                size_t nodes = leaves, l = leaves, depth = 0;
+                while (l > 1) {
+                    l = (l + 1) / 2;
+                    nodes += l;
+                    depth++;
+                }
+                CHECK(gl_row_tree_size(leaves) == 32 * nodes);
+                CHECK(gl_row_tree_depth(leaves) == depth);
             } else if (!device_ok) {
                 uint64_t v[4] = {1, 2, 3, 4};
                 CHECK(gl_ntt(v, 4) == GL_E_DEVICE);
