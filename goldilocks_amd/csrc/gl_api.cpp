@@ -272,11 +272,9 @@ int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
     size_t bytes = n * count * 8;
     DevBuf d(bytes, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
-    GL_HIP(hipMemcpyAsync(d.p, v, bytes, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(d.p, v, bytes, s))) return rc;
     if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s))) return rc;
-    GL_HIP(hipMemcpyAsync(v, d.p, bytes, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return copy_d2h(v, d.p, bytes, s);
 }
 
 int gl_ntt(uint64_t *v, size_t n) { return gl_ntt_batch(v, n, 1, 0); }
@@ -447,12 +445,10 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
     u64 *stage = (u64 *)(dout + out_pad);
-    GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(din, in, len, s))) return rc;
     launch_k12_layer(din, len, dout, stage, false, nullptr, s);
     GL_HIP(hipGetLastError());
-    GL_HIP(hipMemcpyAsync(out, dout, olen, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return copy_d2h(out, dout, olen, s);
 }
 
 // ---------------------------------------------------------------- openings
@@ -529,11 +525,9 @@ int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
     DevBuf d(in_pad + need, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
-    GL_HIP(hipMemcpyAsync(din, in, len, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(din, in, len, s))) return rc;
     if ((rc = merkle_dev_impl(din, len, dout, need, nullptr, s))) return rc;
-    GL_HIP(hipMemcpyAsync(out, dout, need, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return copy_d2h(out, dout, need, s);
 }
 
 int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_nonce, uint64_t *nonce) {
@@ -597,12 +591,10 @@ int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
     DevBuf d(2 * n * 8, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     u64 *da = d.as<u64>(), *db = da + n;
-    GL_HIP(hipMemcpyAsync(da, a, n * 8, hipMemcpyHostToDevice, s));
-    GL_HIP(hipMemcpyAsync(db, b_ntt, n * 8, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(da, a, n * 8, s))) return rc;
+    if ((rc = copy_h2d(db, b_ntt, n * 8, s))) return rc;
     if ((rc = gl_convolve_dev(da, db, n, s))) return rc;
-    GL_HIP(hipMemcpyAsync(a, da, n * 8, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return copy_d2h(a, da, n * 8, s);
 }
 
 }  // extern "C"
@@ -640,13 +632,11 @@ int commit_host(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, ui
     hipStream_t s = thread_stream();
     DevBuf dm(rows * cols * 8, s), dl(ext * cols * 8, s), dt(need + 256, s);
     if (!dm.p || !dl.p || !dt.p) return set_error(GL_E_NOMEM, "device allocation failed");
-    GL_HIP(hipMemcpyAsync(dm.p, m, rows * cols * 8, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(dm.p, m, rows * cols * 8, s))) return rc;
     if ((rc = commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need, nullptr, s, tree)))
         return rc;
-    GL_HIP(hipMemcpyAsync(lde_out, dl.p, ext * cols * 8, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipMemcpyAsync(layers_out, dt.p, need, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    if ((rc = copy_d2h(lde_out, dl.p, ext * cols * 8, s))) return rc;
+    return copy_d2h(layers_out, dt.p, need, s);
 }
 
 }  // namespace
@@ -705,11 +695,9 @@ int gl_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *layers
     DevBuf d(in_pad + need, s);
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
-    GL_HIP(hipMemcpyAsync(din, rows, in_bytes, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(din, rows, in_bytes, s))) return rc;
     if ((rc = row_tree_dev_impl((const u64 *)din, nrows, cols, dout, need, nullptr, s))) return rc;
-    GL_HIP(hipMemcpyAsync(layers_out, dout, need, hipMemcpyDeviceToHost, s));
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    return copy_d2h(layers_out, dout, need, s);
 }
 
 size_t gl_row_tree_depth(size_t leaves) {

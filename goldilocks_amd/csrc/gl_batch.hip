@@ -221,11 +221,11 @@ extern "C" int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k,
     std::vector<uint64_t *> ptrs(k);
     for (size_t i = 0; i < k; i++) {
         ptrs[i] = d.as<u64>() + off[i];
-        if (n[i]) GL_HIP(hipMemcpyAsync(ptrs[i], v[i], n[i] * 8, hipMemcpyHostToDevice, s));
+        if (n[i] && (rc = copy_h2d(ptrs[i], v[i], n[i] * 8, s))) return rc;
     }
     if ((rc = gl_ntt_mixed_dev(ptrs.data(), n, k, inverse, s))) return rc;
     for (size_t i = 0; i < k; i++)
-        if (n[i]) GL_HIP(hipMemcpyAsync(v[i], ptrs[i], n[i] * 8, hipMemcpyDeviceToHost, s));
+        if (n[i] && (rc = copy_d2h(v[i], ptrs[i], n[i] * 8, s, false))) return rc;
     GL_HIP(hipStreamSynchronize(s));
     return GL_OK;
 }

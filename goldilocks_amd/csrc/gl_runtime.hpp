@@ -61,6 +61,15 @@ struct DevBuf {
 
 hipStream_t thread_stream();
 
+// ---- host <-> device copies of the host-buffer entry points ----
+// Large copies are staged through pinned chunks (gl_hostio.cpp).
+// copy_h2d: returns once h_src may be reused; the device side completes in
+// stream order on s.  copy_d2h: ordered after the work on s; returns once
+// h_dst holds the bytes (sync = false: a copy too small to be staged is only
+// enqueued, and the caller synchronizes s before reading h_dst).
+int copy_h2d(void *d_dst, const void *h_src, size_t bytes, hipStream_t s);
+int copy_d2h(void *h_dst, const void *d_src, size_t bytes, hipStream_t s, bool sync = true);
+
 // ---- twiddles (device tables, Montgomery form, cached) ----
 using TableRef = std::shared_ptr<const u64>;
 using Keep = std::vector<TableRef>;
