@@ -23,6 +23,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <unistd.h>
 #include <vector>
 
 #include "gl_runtime.hpp"
@@ -108,11 +109,13 @@ struct CopyPool {
     std::condition_variable cv;
     std::deque<std::function<void()>> q;
     int threads = 0;
+    pid_t pid = 0;  // the process whose threads these are (a forked child has none)
 };
 
 CopyPool *copy_pool() {
     static CopyPool *pool = [] {
         CopyPool *c = new CopyPool();
+        c->pid = getpid();
         const unsigned hw = std::thread::hardware_concurrency();
         const int want = hw > 1 ? (int)std::min(hw - 1, 7u) : 0;
         for (int i = 0; i < want; i++) {
@@ -142,7 +145,8 @@ CopyPool *copy_pool() {
 // memcpy split over the calling thread and the pool's threads.
 void par_memcpy(void *dst, const void *src, size_t n) {
     CopyPool *c = copy_pool();
-    const int parts = (int)std::min<size_t>((size_t)c->threads + 1, n / kPart);
+    const int helpers = c->pid == getpid() ? c->threads : 0;
+    const int parts = (int)std::min<size_t>((size_t)helpers + 1, n / kPart);
     if (parts <= 1) {
         std::memcpy(dst, src, n);
         return;
