@@ -7,7 +7,8 @@ oracle's port of the reference algorithm on this host's cores, bounded
 sample).  Extra workloads: --workload batch (mixed-size batched NTTs,
 batched vs one-launch-sequence-per-transform, with p50/p99 latency),
 --workload commit (coset LDE + K12 Merkle commit), --workload serve (open-loop
-Poisson NTT/commit jobs through the continuous-batching server).
+Poisson NTT/commit jobs through the continuous-batching server), --workload
+permute (the reference's transpose and bit_reverse on HBM-resident words).
 
 The default line also carries the metric's second half, "mixed_batch": the
 1024-transform mixed-size batch (n = 2^k * f, k in [10, 20], f in
@@ -33,7 +34,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--log-n", type=int, default=24)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve", "permute"])
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
     ap.add_argument("--no-host-buffer", action="store_true",
                     help="ntt: skip the host-array (PCIe) timing, e.g. under a profiler")
@@ -299,6 +300,9 @@ def main():
     elif args.workload == "commit":
         from goldilocks_amd.benchmarks import bench_commit
         out = bench_commit(args, torch, gl)
+    elif args.workload == "permute":
+        from goldilocks_amd.benchmarks import bench_permute
+        out = bench_permute(args, torch, gl)
     else:
         from goldilocks_amd.benchmarks import bench_serve
         out = bench_serve(args, torch, gl)

@@ -123,6 +123,11 @@ def lib():
             "gl_row_tree_dev": (i32, [vp, sz, sz, vp, sz, psz, vp]),
             "gl_row_open_dev": (i32, [vp, sz, vp, sz, vp, vp]),
             "gl_row_verify_dev": (i32, [vp, sz, sz, vp, sz, vp, vp, vp, vp]),
+            "gl_transpose": (i32, [vp, sz, sz]),
+            "gl_transpose_dev": (i32, [vp, sz, sz, vp]),
+            "gl_transpose_oop_dev": (i32, [vp, vp, sz, sz, vp]),
+            "gl_bit_reverse": (i32, [vp, sz]),
+            "gl_bit_reverse_dev": (i32, [vp, sz, vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -331,6 +336,37 @@ def ntt_batch_mixed(arrays, inverse=False, batched=True):
         ptrs = (ctypes.c_void_p * k)(*[_host_u64(a).ctypes.data for a in arrays])
         _check(lib().gl_ntt_batch_mixed(ptrs, ns, k, int(inverse)))
     return arrays
+
+
+# ---------------------------------------------------------------- permutations
+def transpose(values, rows, cols, out=None):
+    """permute::transpose (ntt/src/permute/mod.rs:53-58): the flat rows x
+    cols row-major matrix becomes its cols x rows transpose, in place (numpy
+    array or contiguous GPU tensor).  out (GPU tensor, distinct from values):
+    write the transpose there instead and leave values untouched."""
+    if _is_torch(values):
+        if out is not None:
+            _check(lib().gl_transpose_oop_dev(_dev_ptr(values), _dev_ptr(out), rows, cols, _stream_ptr(values)))
+            return out
+        _check(lib().gl_transpose_dev(_dev_ptr(values), rows, cols, _stream_ptr(values)))
+        return values
+    if out is not None:
+        raise ValueError("out= is for GPU tensors")
+    v = _host_u64(values)
+    _check(lib().gl_transpose(v.ctypes.data_as(ctypes.c_void_p), rows, cols))
+    return values
+
+
+def bit_reverse(values):
+    """permute::bit_reverse (ntt/src/permute/mod.rs:71-74): values[i] <-
+    values[rev(i)] in place; the length must be a power of two."""
+    n = _len(values)
+    if _is_torch(values):
+        _check(lib().gl_bit_reverse_dev(_dev_ptr(values), n, _stream_ptr(values)))
+    else:
+        v = _host_u64(values)
+        _check(lib().gl_bit_reverse(v.ctypes.data_as(ctypes.c_void_p), n))
+    return values
 
 
 def fill_random(tensor, seed):

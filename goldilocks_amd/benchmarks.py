@@ -289,3 +289,47 @@ def bench_serve(args, torch, gl):
         "rejected": int(st["rejected"] - base["rejected"]),
         "peak_bytes": int(st["peak_bytes"]),
     }
+
+
+def bench_permute(args, torch, gl):
+    """The reference's public permutations on device-resident words
+    (permute::transpose, permute::bit_reverse): median time per call and
+    the rate against the 16 bytes per word they must move (read + write)."""
+    stream = torch.cuda.current_stream()
+    cases = []
+
+    def case(name, words, fn, note):
+        times = _timed_reps(torch, stream, fn, min_total_ms=300.0, min_reps=10, max_reps=200)
+        ms = sorted(times)[len(times) // 2]
+        gbs = 16 * words / (ms * 1e-3) / 1e9
+        cases.append({"case": name, "words": words, "ms_p50": round(ms, 4), "algorithmic_gbs": round(gbs, 1),
+                      "hbm_frac": round(gbs / 8000.0, 4), "path": note})
+
+    n = 4096
+    sq = torch.empty(n * n, dtype=torch.int64, device="cuda")
+    gl.fill_random(sq, 3)
+    case(f"transpose {n}x{n} in place", n * n, lambda: gl.transpose(sq, n, n),
+         "tile pairs swapped through LDS, one read and one write per word")
+    r, c = 8192, 2048
+    src = torch.empty(r * c, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    gl.fill_random(src, 4)
+    case(f"transpose {r}x{c} out of place", r * c, lambda: gl.transpose(src, r, c, out=dst),
+         "32x32 tiles through LDS")
+    case(f"transpose {r}x{c} in place", r * c, lambda: gl.transpose(src, r, c),
+         "out of place into pool scratch, then a device copy back (32 B per word moved)")
+    k = 24
+    v = torch.empty(1 << k, dtype=torch.int64, device="cuda")
+    gl.fill_random(v, 5)
+    case(f"bit_reverse 2^{k}", 1 << k, lambda: gl.bit_reverse(v),
+         "tile pairs (middle bits b, rev(b)) swapped through LDS")
+    copy_times = _timed_reps(torch, stream, lambda: dst.copy_(src), min_total_ms=300.0, min_reps=10, max_reps=200)
+    copy_ms = sorted(copy_times)[len(copy_times) // 2]
+    head = cases[0]
+    return {
+        "metric": "GB/s of the reference's permutations on HBM-resident u64 words (16 B per word)",
+        "value": head["algorithmic_gbs"], "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "u64", "data": "synthetic: seeded uniform [0,p) words, device-resident",
+        "config": {"workload": head["case"]}, "cases": cases,
+        "copy_gbs": round(16 * r * c / (copy_ms * 1e-3) / 1e9, 1),
+    }

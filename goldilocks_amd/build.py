@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgoldilocks_amd.so")
 SOURCES = ["gl_ntt_kernels.hip"] + [f"gl_variants_{k}.hip" for k in range(8)] + [
-    "gl_rader.hip", "gl_k12.hip", "gl_batch.hip", "gl_runtime.cpp", "gl_api.cpp", "gl_hostio.cpp", "gl_server.cpp", "gl_noalloc.hip"]
+    "gl_rader.hip", "gl_k12.hip", "gl_permute.hip", "gl_batch.hip", "gl_runtime.cpp", "gl_api.cpp", "gl_hostio.cpp", "gl_server.cpp", "gl_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

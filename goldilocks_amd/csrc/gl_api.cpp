@@ -582,6 +582,108 @@ int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stre
     return ntt_dev_impl(d_a, n, 1, 1, true, s);
 }
 
+// ---------------------------------------------------------------- permutations
+
+// Shape checks shared by the transposes: GL_OK for an empty or valid shape.
+// One workgroup per 32 x 32 tile, at most 2^24 - 1 workgroups per launch.
+static int transpose_check(size_t rows, size_t cols) {
+    if (words_overflow(rows, cols)) return set_error(GL_E_ARG, "transpose shape overflows the address space");
+    if (((u64)rows + 31) / 32 * (((u64)cols + 31) / 32) >= (1ull << 24))
+        return set_error(GL_E_SIZE, "transpose: more than 2^34 words");
+    return GL_OK;
+}
+
+int gl_transpose_oop_dev(const uint64_t *d_src, uint64_t *d_dst, size_t rows, size_t cols, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = transpose_check(rows, cols))) return rc;
+    if (rows == 0 || cols == 0) return GL_OK;
+    if (!d_src || !d_dst) return set_error(GL_E_ARG, "null device pointer");
+    if (d_src == d_dst) return set_error(GL_E_ARG, "out-of-place transpose needs distinct buffers");
+    launch_transpose(d_src, d_dst, rows, cols, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int gl_transpose_dev(uint64_t *d, size_t rows, size_t cols, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = transpose_check(rows, cols))) return rc;
+    if (rows == 0 || cols == 0 || rows == 1 || cols == 1) return GL_OK;  // a vector is its own transpose
+    if (!d) return set_error(GL_E_ARG, "null device pointer");
+    hipStream_t s = as_stream(stream);
+    if (rows == cols) {
+        launch_transpose_square(d, rows, s);
+        GL_HIP(hipGetLastError());
+        return GL_OK;
+    }
+    const size_t bytes = rows * cols * 8;
+    DevBuf tmp(bytes, s);  // stream ordered: back to the pool after the copy
+    if (!tmp.p) return set_error(GL_E_NOMEM, "transpose scratch allocation failed");
+    launch_transpose(d, tmp.as<u64>(), rows, cols, s);
+    GL_HIP(hipGetLastError());
+    GL_HIP(hipMemcpyAsync(d, tmp.p, bytes, hipMemcpyDeviceToDevice, s));
+    return GL_OK;
+}
+
+int gl_transpose(uint64_t *m, size_t rows, size_t cols) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = transpose_check(rows, cols))) return rc;
+    if (rows == 0 || cols == 0 || rows == 1 || cols == 1) return GL_OK;
+    if (!m) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    const size_t bytes = rows * cols * 8;
+    DevBuf a(bytes, s), b(rows == cols ? 0 : bytes, s);
+    if (!a.p || (rows != cols && !b.p)) return set_error(GL_E_NOMEM, "device allocation failed");
+    if ((rc = copy_h2d(a.p, m, bytes, s))) return rc;
+    if (rows == cols) {
+        launch_transpose_square(a.as<u64>(), rows, s);
+        GL_HIP(hipGetLastError());
+        return copy_d2h(m, a.p, bytes, s);
+    }
+    launch_transpose(a.as<u64>(), b.as<u64>(), rows, cols, s);
+    GL_HIP(hipGetLastError());
+    return copy_d2h(m, b.p, bytes, s);
+}
+
+static int bit_reverse_log2(size_t n, int *k) {
+    if (n == 0 || (n & (n - 1))) return set_error(GL_E_SIZE, "bit_reverse: length must be a power of two");
+    if (n > (size_t(1) << 32)) return set_error(GL_E_SIZE, "bit_reverse: length above 2^32");
+    int b = 0;
+    while ((size_t(1) << b) < n) b++;
+    *k = b;
+    return GL_OK;
+}
+
+int gl_bit_reverse_dev(uint64_t *d, size_t n, void *stream) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    int k = 0;
+    if ((rc = bit_reverse_log2(n, &k))) return rc;
+    if (n == 1) return GL_OK;
+    if (!d) return set_error(GL_E_ARG, "null device pointer");
+    launch_bit_reverse(d, n, k, as_stream(stream));
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int gl_bit_reverse(uint64_t *v, size_t n) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    int k = 0;
+    if ((rc = bit_reverse_log2(n, &k))) return rc;
+    if (n == 1) return GL_OK;
+    if (!v) return set_error(GL_E_ARG, "null pointer");
+    hipStream_t s = thread_stream();
+    DevBuf d(n * 8, s);
+    if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    if ((rc = copy_h2d(d.p, v, n * 8, s))) return rc;
+    launch_bit_reverse(d.as<u64>(), n, k, s);
+    GL_HIP(hipGetLastError());
+    return copy_d2h(v, d.p, n * 8, s);
+}
+
 int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
     int rc = ensure_init();
     if (rc) return rc;

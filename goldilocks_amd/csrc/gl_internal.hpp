@@ -111,6 +111,12 @@ const PassVariant *rader257_variant();
 
 // Elementwise kernels (gl_ntt_kernels.hip).
 void launch_fill_random(u64 *v, u64 n, u64 seed, hipStream_t s);
+// Permutations (gl_permute.hip): out (cols x rows) = transpose of in (rows x
+// cols), in != out; an n x n matrix transposed in place; v[i] <- v[rev(i)]
+// for n = 2^k in place.
+void launch_transpose(const u64 *in, u64 *out, u64 rows, u64 cols, hipStream_t s);
+void launch_transpose_square(u64 *m, u64 n, hipStream_t s);
+void launch_bit_reverse(u64 *v, u64 n, int k, hipStream_t s);
 void launch_twiddle_matrix(u64 *out, u64 count, u64 R, const u64 *lo, const u64 *hi, u32 lo_bits,
                            hipStream_t s);
 void launch_rader_big_gather(const PassArgs &a, const u32 *permk, u64 *C, u64 *x0, hipStream_t s);

@@ -209,6 +209,25 @@ int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_no
 int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n);
 int gl_convolve_dev(uint64_t *d_a, const uint64_t *d_b_ntt, size_t n, void *stream);
 
+/* ---- Permutations (ntt/src/permute/mod.rs) --------------------------------
+ * Of u64 words (the reference's are generic over Copy types).
+ * gl_transpose(_dev): permute::transpose (mod.rs:53-58, the semantics of
+ * transpose_copy, mod.rs:60-67): the rows x cols row-major matrix becomes its
+ * cols x rows transpose in place.  Square matrices swap tile pairs in place;
+ * other shapes go through device scratch (pool) and a copy back.
+ * gl_transpose_oop_dev: the same move from src into dst (dst != src, no
+ * scratch; the reference has no out-of-place form).  rows * cols == 0 is a
+ * no-op; a shape whose byte size overflows is GL_E_ARG.
+ * gl_bit_reverse(_dev): permute::bit_reverse (mod.rs:71-74), v[i] <- v[rev(i)]
+ * with rev reversing the log2(n) low bits (permutation.rs:9-16); n must be a
+ * power of two (the reference asserts it, permutation.rs:13): GL_E_SIZE
+ * otherwise, n = 0 included. */
+int gl_transpose(uint64_t *m, size_t rows, size_t cols);
+int gl_transpose_dev(uint64_t *d, size_t rows, size_t cols, void *stream);
+int gl_transpose_oop_dev(const uint64_t *d_src, uint64_t *d_dst, size_t rows, size_t cols, void *stream);
+int gl_bit_reverse(uint64_t *v, size_t n);
+int gl_bit_reverse_dev(uint64_t *d, size_t n, void *stream);
+
 /* ---- Merkle openings (the step after the commit) --------------------------
  * No reference function: the reference returns the layers (pcs/src/k12/
  * mod.rs:41-60) and stops.  An opening of leaf block b (the 160-byte block

@@ -119,6 +119,15 @@ int or_commit(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
 int or_commit_rows(const uint64_t *m, size_t rows, size_t cols, unsigned blowup,
                    uint64_t *lde_out, uint8_t *layers_out, size_t cap, size_t *out_len);
 
+/* ---- permutations (ntt/src/permute/mod.rs, permutation.rs) ------------------
+ * Pinned: the reference's KATs (mod.rs test_copy_2_3 / test_copy_3_3,
+ * permutation.rs test_transpose) and numpy's transpose / an independent
+ * bit-reversal on random shapes (tests/test_oracle_permute.py). */
+size_t or_transpose_index(size_t width, size_t height, size_t i);  /* permutation.rs:1-4 */
+int or_transpose(uint64_t *m, size_t rows, size_t cols);         /* mod.rs:53-67 */
+size_t or_bit_reverse_index(size_t size, size_t i);              /* permutation.rs:9-16 */
+int or_bit_reverse(uint64_t *v, size_t n);                        /* mod.rs:71-74; -1 unless a power of two */
+
 /* ---- seeded inputs ----------------------------------------------------------
  * Word i = the first splitmix64 output of the stream seeded with
  * seed + i*0x9e3779b97f4a7c15 that is < p (re-mixing on rejection), so any

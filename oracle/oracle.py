@@ -56,6 +56,8 @@ def lib():
             "or_process_layer": (None, [cp, sz, vp]),
             "or_merkle_tree": (i32, [cp, sz, vp, sz, ctypes.POINTER(sz)]),
             "or_merkle_tree_size": (sz, [sz]),
+            "or_transpose_index": (sz, [sz, sz, sz]), "or_transpose": (i32, [vp, sz, sz]),
+            "or_bit_reverse_index": (sz, [sz, sz]), "or_bit_reverse": (i32, [vp, sz]),
             "or_pow_verify": (i32, [cp, u64, u64]),
             "or_pow_solve": (u64, [cp, u64]),
             "or_commit": (i32, [vp, sz, sz, ctypes.c_uint, vp, vp, sz, ctypes.POINTER(sz)]),
@@ -120,6 +122,27 @@ def ntt_batch(v, n, inverse=False):
     v = _u64(v)
     assert v.size % n == 0 if n else v.size == 0
     return _run(lib().or_ntt_batch, v, n, v.size // n if n else 0, int(bool(inverse)))
+
+
+# ---- permutations ----
+def transpose_index(width, height, i): return lib().or_transpose_index(width, height, i)
+def bit_reverse_index(size, i): return lib().or_bit_reverse_index(size, i)
+
+
+def transpose(m, rows, cols):
+    """A rows x cols row-major matrix (flat) -> its cols x rows transpose (flat)."""
+    v = _u64(m).reshape(-1).copy()
+    assert v.size == rows * cols
+    if lib().or_transpose(_ptr(v), rows, cols) != 0:
+        raise MemoryError("or_transpose")
+    return v
+
+
+def bit_reverse(v):
+    v = _u64(v).copy()
+    if lib().or_bit_reverse(_ptr(v), v.size) != 0:
+        raise ValueError("bit_reverse: length must be a power of two")
+    return v
 
 
 def eval_dft(x, k):

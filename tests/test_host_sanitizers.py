@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CLANGXX = "/opt/rocm/llvm/bin/clang++"
 INSTRUMENTED = ["gl_runtime.cpp", "gl_api.cpp", "gl_hostio.cpp", "gl_server.cpp", "gl_batch.hip"]
 PLAIN = ["gl_ntt_kernels.hip"] + [f"gl_variants_{k}.hip" for k in range(8)] + [
-    "gl_rader.hip", "gl_k12.hip", "gl_noalloc.hip"]
+    "gl_rader.hip", "gl_k12.hip", "gl_permute.hip", "gl_noalloc.hip"]
 
 
 def _build(kind):
