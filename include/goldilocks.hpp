@@ -67,6 +67,16 @@ inline void circular(std::vector<Field> &a, const std::vector<Field> &b) {
     check(gl_convolve(a.data(), b.data(), a.size()));
 }
 
+// permute::transpose (ntt/src/permute/mod.rs:53-58): rows x cols -> cols x rows in place.
+namespace permute {
+inline void transpose(std::vector<Field> &matrix, std::size_t rows, std::size_t cols) {
+    if (matrix.size() != rows * cols) throw Error(GL_E_ARG, "matrix length is not rows * cols");
+    check(gl_transpose(matrix.data(), rows, cols));
+}
+// permute::bit_reverse (mod.rs:71-74): the length must be a power of two.
+inline void bit_reverse(std::vector<Field> &values) { check(gl_bit_reverse(values.data(), values.size())); }
+}  // namespace permute
+
 }  // namespace ntt
 
 namespace pcs {

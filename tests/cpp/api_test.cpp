@@ -78,6 +78,15 @@ int main() {
         CHECK((a == std::vector<Field>{mont(31), mont(31), mont(28)}));
     }
     {
+        // permute/mod.rs test_copy_2_3, and a bit reversal of 8 words
+        std::vector<Field> m{0, 1, 2, 3, 4, 5};
+        ntt::permute::transpose(m, 2, 3);
+        CHECK((m == std::vector<Field>{0, 3, 1, 4, 2, 5}));
+        std::vector<Field> r{0, 1, 2, 3, 4, 5, 6, 7};
+        ntt::permute::bit_reverse(r);
+        CHECK((r == std::vector<Field>{0, 4, 2, 6, 1, 5, 3, 7}));
+    }
+    {
         auto h = pcs::k12<32>(nullptr, 0);
         const std::uint8_t e[4] = {0x1a, 0xc2, 0xd4, 0x50};
         CHECK(std::memcmp(h.data(), e, 4) == 0);
