@@ -293,14 +293,23 @@ def bench_serve(args, torch, gl):
 
 def bench_permute(args, torch, gl):
     """The reference's public permutations on device-resident words
-    (permute::transpose, permute::bit_reverse): median time per call and
-    the rate against the 16 bytes per word they must move (read + write)."""
+    (permute::transpose, permute::bit_reverse): median time per call (over
+    repetitions of 20 back-to-back calls) and the rate against the 16 bytes
+    per word they must move (read + write)."""
     stream = torch.cuda.current_stream()
     cases = []
 
+    calls = 20  # back-to-back calls per timed repetition (a call is ~40-90 us)
+
+    def repeated(fn):
+        def run():
+            for _ in range(calls):
+                fn()
+        return run
+
     def case(name, words, fn, note):
-        times = _timed_reps(torch, stream, fn, min_total_ms=300.0, min_reps=10, max_reps=200)
-        ms = sorted(times)[len(times) // 2]
+        times = _timed_reps(torch, stream, repeated(fn), min_total_ms=300.0, min_reps=10, max_reps=200)
+        ms = sorted(times)[len(times) // 2] / calls
         gbs = 16 * words / (ms * 1e-3) / 1e9
         cases.append({"case": name, "words": words, "ms_p50": round(ms, 4), "algorithmic_gbs": round(gbs, 1),
                       "hbm_frac": round(gbs / 8000.0, 4), "path": note})
@@ -323,8 +332,9 @@ def bench_permute(args, torch, gl):
     gl.fill_random(v, 5)
     case(f"bit_reverse 2^{k}", 1 << k, lambda: gl.bit_reverse(v),
          "tile pairs (middle bits b, rev(b)) swapped through LDS")
-    copy_times = _timed_reps(torch, stream, lambda: dst.copy_(src), min_total_ms=300.0, min_reps=10, max_reps=200)
-    copy_ms = sorted(copy_times)[len(copy_times) // 2]
+    copy_times = _timed_reps(torch, stream, repeated(lambda: dst.copy_(src)), min_total_ms=300.0, min_reps=10,
+                             max_reps=200)
+    copy_ms = sorted(copy_times)[len(copy_times) // 2] / calls
     head = cases[0]
     return {
         "metric": "GB/s of the reference's permutations on HBM-resident u64 words (16 B per word)",
