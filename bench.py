@@ -131,6 +131,83 @@ def cpu_baseline_ntt(n, seconds):
                     "~0.010 Gelem/s on EPYC 7R13 and ~0.031 on M1 Max (bench/analysis.ipynb plots)"}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
+def _reference_rule(fn, budget, max_reps=50):
+    """Median seconds of fn over repeats until the accumulated time passes
+    budget (bench/src/main.rs:61-95); returns (median, repeats)."""
+    times = []
+    while not times or (sum(times) < budget and len(times) < max_reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), len(times)
+
+
+def _cpu_fields(threads, usable, model):
+    return {"cores": threads, "kind": "port", "cpu_model": model, "host_logical_cpus": usable}
+
+
+def cpu_baseline_commit(seconds):
+    """The oracle's commit (the reference's intt / ntt plan tree per column
+    and its merkle_tree) on every usable host core, on a bounded sample of
+    the commit workload: 2^16 of its 2^20 rows, same 64 columns and blowup."""
+    if seconds <= 0:
+        return None
+    oracle = _oracle()
+    model, usable, threads = host_cpu()
+    rows, cols, blowup = 1 << 16, 64, 2
+    m = oracle.fill_random(rows * cols, 5).reshape(rows, cols)
+    oracle.set_threads(threads)
+    med, k = _reference_rule(lambda: oracle.commit(m, blowup), seconds)
+    oracle.set_threads(1)
+    return {"value": round(rows * blowup * cols / med / 1e9, 5), "unit": "Gelem/s",
+            "sample": f"commit of {rows} x {cols} words, blowup 2 (1/16 of the workload's rows), "
+                      f"median of {k}", **_cpu_fields(threads, usable, model)}
+
+
+def cpu_baseline_batch(seconds, batch_k=(10, 20), count=1024, seed=7, every=32):
+    """The oracle's forward NTTs on every usable host core over a bounded
+    sample of the mixed batch: every 32nd of its 1024 lengths."""
+    if seconds <= 0:
+        return None
+    oracle = _oracle()
+    from goldilocks_amd.benchmarks import mixed_sizes
+    model, usable, threads = host_cpu()
+    sizes = mixed_sizes(count, seed, *batch_k)[::every]
+    xs = [oracle.fill_random(n, 1000 + i) for i, n in enumerate(sizes)]
+    oracle.set_threads(threads)
+
+    def run():
+        for x in xs:
+            oracle.ntt(x)
+    med, k = _reference_rule(run, seconds, max_reps=5)
+    oracle.set_threads(1)
+    return {"value": round(sum(sizes) / med / 1e9, 5), "unit": "Gelem/s",
+            "sample": f"{len(sizes)} of the batch's lengths (one in every {every}, {sum(sizes)} words), median of {k}",
+            **_cpu_fields(threads, usable, model)}
+
+
+def cpu_baseline_permute(seconds):
+    """The oracle's transpose (the reference's transpose_copy loop) of a
+    4096 x 4096 matrix on one host core (the reference's copy loop is
+    serial), 16 bytes per word."""
+    if seconds <= 0:
+        return None
+    oracle = _oracle()
+    model, usable, _ = host_cpu()
+    n = 4096
+    m = oracle.fill_random(n * n, 3)
+    med, k = _reference_rule(lambda: oracle.transpose(m, n, n), seconds, max_reps=20)
+    return {"value": round(16 * n * n / med / 1e9, 3), "unit": "GB/s",
+            "sample": f"transpose of {n} x {n} words, median of {k}",
+            **_cpu_fields(1, usable, model)}
+
+
 def warm_up(fn, steps, torch, min_seconds=1.0, max_seconds=4.0):
     """`steps` untimed calls, then more until at least min_seconds of wall
     time have passed (GPU clocks settle under load), never past
@@ -297,12 +374,15 @@ def main():
     elif args.workload == "batch":
         from goldilocks_amd.benchmarks import bench_batch
         out = bench_batch(args, torch, gl)
+        out["cpu_baseline"] = cpu_baseline_batch(args.cpu_seconds, args.batch_k)
     elif args.workload == "commit":
         from goldilocks_amd.benchmarks import bench_commit
         out = bench_commit(args, torch, gl)
+        out["cpu_baseline"] = cpu_baseline_commit(args.cpu_seconds)
     elif args.workload == "permute":
         from goldilocks_amd.benchmarks import bench_permute
         out = bench_permute(args, torch, gl)
+        out["cpu_baseline"] = cpu_baseline_permute(min(args.cpu_seconds, 4.0))
     else:
         from goldilocks_amd.benchmarks import bench_serve
         out = bench_serve(args, torch, gl)
