@@ -253,6 +253,8 @@ uint64_t gl_next_size(uint64_t n) {
 
 // ---------------------------------------------------------------- NTT
 
+void gl_debug_fail_allocs(int count) { set_fail_allocs(count); }
+
 int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream) {
     return ntt_dev_impl(d, n, 1, count, inverse != 0, as_stream(stream));
 }

@@ -107,7 +107,15 @@ void recycle_event(hipEvent_t e) {
 }
 }  // namespace
 
+// Allocations still to fail on purpose (gl_debug_fail_allocs): a host-side
+// switch for tests of the error paths; no device work is involved.
+static std::atomic<int> g_fail_allocs{0};
+
+void set_fail_allocs(int count) { g_fail_allocs.store(count > 0 ? count : 0); }
+
 void *pool_alloc(size_t bytes, hipStream_t s) {
+    for (int f = g_fail_allocs.load(); f > 0;)
+        if (g_fail_allocs.compare_exchange_weak(f, f - 1)) return nullptr;
     const size_t sz = round_size(bytes);
     {
         std::lock_guard<std::mutex> lk(g_pool_mu);

@@ -46,6 +46,8 @@ void *pool_alloc(size_t bytes, hipStream_t s);
 void pool_free(void *p, hipStream_t s);
 // Free every idle block whose last use has completed (no device sync).
 void pool_release_idle();
+// The next `count` pool_alloc calls return nullptr (error-path tests).
+void set_fail_allocs(int count);
 
 // A pool block owned for the duration of one enqueue on a stream.
 struct DevBuf {

@@ -168,6 +168,12 @@ void gl_timing_enable(int on);
 size_t gl_timing_pending(void);
 int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *count);
 
+/* Testing aid with no reference counterpart (the reference aborts on
+ * allocation failure): the next `count` device scratch allocations of the
+ * library fail as if hipMalloc had, so callers can exercise the GL_E_NOMEM
+ * paths of every entry point and of the server; 0 clears it. */
+void gl_debug_fail_allocs(int count);
+
 /* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46).
  * Tables still referenced by work being enqueued or by a batch plan stay
  * valid and are freed by a later call; explicit (gl_plan_set) and tuned
