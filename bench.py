@@ -224,6 +224,45 @@ def warm_up(fn, steps, torch, min_seconds=1.0, max_seconds=4.0):
             break
 
 
+def other_sizes(torch, gl, log_sizes=(16, 20, 27), seconds=0.3):
+    """The other lengths of the plain-NTT workloads (2^16 forward + inverse;
+    2^20 and 2^27 forward then inverse, HBM-resident): ms per call over
+    back-to-back calls after a warm-up, with the transform's 16 n bytes
+    against the 8 TB/s peak.  A roundtrip check guards each size."""
+    stream = torch.cuda.current_stream()
+    out = {}
+    for lg in log_sizes:
+        n = 1 << lg
+        x = torch.empty(n, dtype=torch.int64, device="cuda")
+        gl.fill_random(x, 40 + lg)
+        ref = x.clone()
+        res = {}
+        for name, fn in (("forward", gl.ntt), ("inverse", gl.intt)):
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < seconds:
+                fn(x)
+                torch.cuda.synchronize()
+            reps = max(5, min(2000, int(2e-3 / (1e-9 * n) + 5)))  # ~2 ms of calls at 1 ns/word
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(x)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[name] = {"ms": round(ms, 5), "gelem_s": round(n / (ms * 1e-3) / 1e9, 2),
+                         "hbm_frac": round(16 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        x.copy_(ref)
+        gl.ntt(x)
+        gl.intt(x)
+        torch.cuda.synchronize()
+        res["roundtrip_exact"] = bool(torch.equal(x, ref))  # intt(ntt(x)) == x
+        out[f"2^{lg}"] = res
+        del x, ref
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_ntt(args, torch, gl):
     n = 1 << args.log_n
     x = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -365,6 +404,8 @@ def main():
     if args.workload == "ntt":
         out = bench_ntt(args, torch, gl)
         torch.cuda.empty_cache()
+        # the plain-NTT workloads' other lengths (2^16, 2^20, 2^27)
+        out["other_sizes"] = other_sizes(torch, gl)
         if not args.no_batch:
             # the metric's second half: p50/p99 of the mixed-size batch
             from goldilocks_amd.benchmarks import batch_summary
