@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve", "permute"])
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
+    ap.add_argument("--slots", type=int, default=8, help="serve: server slots (streams), 1-16")
     ap.add_argument("--no-host-buffer", action="store_true",
                     help="ntt: skip the host-array (PCIe) timing, e.g. under a profiler")
     ap.add_argument("--batch-k", type=int, nargs=2, default=[10, 20], metavar=("LO", "HI"),

@@ -215,7 +215,8 @@ def bench_serve(args, torch, gl):
             bufs.append((m, lde, layers))
     torch.cuda.synchronize()
     budget = 32 << 30
-    with gl.Server(hbm_budget_bytes=budget, slots=4) as srv:
+    slots = getattr(args, "slots", 8) or 8
+    with gl.Server(hbm_budget_bytes=budget, slots=slots) as srv:
         # warm the plans and twiddle tables of every shape once
         seen = set()
         for (kind, n, c), b in zip(jobs, bufs):
@@ -272,7 +273,7 @@ def bench_serve(args, torch, gl):
         "data": "synthetic: seeded uniform [0,p) words, device-resident buffers",
         "config": {"workload": f"{count} jobs, 75% NTT (n = 2^k*f, k in [12,22], f in {list(SERVE_FACTORS)}, "
                                "1-64 transforms) and 25% commits (2^12..2^18 rows x 1-64 cols, blowup 2), "
-                               "n*cols <= 2^24", "qps": round(qps, 1), "slots": 4,
+                               "n*cols <= 2^24", "qps": round(qps, 1), "slots": slots,
                    "hbm_budget_bytes": budget},
         "latency_ms": {"p50": round(lat[len(lat) // 2], 4), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 4),
                        "max": round(lat[-1], 4)},
