@@ -687,7 +687,9 @@ class Server:
     device.  Buffers (numpy for host jobs, GPU tensors for device jobs) must
     stay alive until the job is done."""
 
-    def __init__(self, hbm_budget_bytes=0, slots=4):
+    def __init__(self, hbm_budget_bytes=0, slots=8):
+        # 8 slots: at saturation the mixed serving workload runs 42 / 62 /
+        # 73 / 77 Gelem/s with 1 / 2 / 4 / 8 (16 adds nothing)
         h = lib().gl_server_create(hbm_budget_bytes, slots)
         if not h:
             raise GoldilocksError(GL_E_DEVICE, lib().gl_last_error().decode())
