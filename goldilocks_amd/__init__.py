@@ -124,6 +124,7 @@ def lib():
             "gl_row_open_dev": (i32, [vp, sz, vp, sz, vp, vp]),
             "gl_row_verify_dev": (i32, [vp, sz, sz, vp, sz, vp, vp, vp, vp]),
             "gl_debug_fail_allocs": (None, [i32]),
+            "gl_debug_fail_allocs_after": (None, [i32, i32]),
             "gl_transpose": (i32, [vp, sz, sz]),
             "gl_transpose_dev": (i32, [vp, sz, sz, vp]),
             "gl_transpose_oop_dev": (i32, [vp, vp, sz, sz, vp]),

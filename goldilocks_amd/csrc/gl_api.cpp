@@ -1,11 +1,13 @@
 // C ABI (include/goldilocks_gpu.h): host-buffer and device-buffer entry
 // points for the NTT, K12 layers, Merkle trees and the commit.
 #include <algorithm>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gl_field.hpp"
@@ -217,6 +219,139 @@ int commit_enqueue(const u64 *d_m, size_t rows, size_t cols, unsigned blowup, u6
     return merkle_dev_impl((const uint8_t *)d_lde, ext * cols * 8, d_layers, need, nullptr, s);
 }
 
+// ---------------------------------------------------------------- pipelined host batch
+constexpr size_t kPipeChunk = size_t(32) << 20;  // bytes per chunk (whole transforms, at least one)
+constexpr int kPipeUnavailable = -1;              // no helper thread or streams: use the serial path
+
+// A host batch of several chunks (a chunk: the whole transforms that fit in
+// kPipeChunk bytes, or one larger transform) runs as a pipeline over two
+// device buffers: a helper
+// thread copies chunk i + 1 to the device on one stream while the device
+// transforms chunk i on the calling thread's stream and the calling thread
+// copies chunk i - 1 back on a third, so the two copy directions (separate
+// DMA engines) and the transforms overlap.  A buffer is refilled only after
+// its chunk is back in host memory.
+
+struct PipeState {
+    std::mutex mu;
+    std::condition_variable cv;
+    bool slot_free[2] = {true, true};
+    size_t ready = 0;  // chunks whose copy to the device is enqueued
+    bool abort = false;
+    int in_rc = GL_OK;
+    std::string in_msg;
+};
+
+struct PipeEvents {
+    hipEvent_t e[5] = {};
+    ~PipeEvents() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+int ntt_batch_pipelined(uint64_t *v, size_t n, size_t count, bool inverse) {
+    const size_t per = std::max<size_t>(1, kPipeChunk / (n * 8));  // transforms per chunk
+    const size_t nchunks = (count + per - 1) / per;
+    hipStream_t s = thread_stream(), s_in = thread_aux_stream(0), s_out = thread_aux_stream(1);
+    if (!s || !s_in || !s_out) return kPipeUnavailable;
+    int dev = 0;
+    GL_HIP(hipGetDevice(&dev));
+    PlanRef plan;  // builds the plan (and reports an unsupported length) before any copy
+    int rc = get_plan(n, &plan);
+    if (rc) return rc;
+    const size_t cbytes = per * n * 8;
+    DevBuf b0(cbytes, s), b1(cbytes, s);
+    if (!b0.p || !b1.p) return set_error(GL_E_NOMEM, "device allocation failed");
+    u64 *buf[2] = {b0.as<u64>(), b1.as<u64>()};
+    PipeEvents ev;  // e[0]: buffers allocated (s); e[1..2]: copy in done (s_in); e[3..4]: transformed (s)
+    for (hipEvent_t &x : ev.e) GL_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    hipEvent_t ev_alloc = ev.e[0], *ev_in = ev.e + 1, *ev_done = ev.e + 3;
+    GL_HIP(hipEventRecord(ev_alloc, s));
+    PipeState st;
+    auto copy_in = [&] {
+        int irc = GL_OK;
+        try {
+            hipError_t e = hipSetDevice(dev);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s_in, ev_alloc, 0);
+            if (e != hipSuccess) irc = hip_error(e, "pipelined batch: copy stream");
+            for (size_t i = 0; i < nchunks && irc == GL_OK; i++) {
+                const int slot = (int)(i & 1);
+                {
+                    std::unique_lock<std::mutex> lk(st.mu);
+                    st.cv.wait(lk, [&] { return st.slot_free[slot] || st.abort; });
+                    if (st.abort) return;
+                    st.slot_free[slot] = false;
+                }
+                const size_t cnt = std::min(per, count - i * per);
+                irc = copy_h2d(buf[slot], v + i * per * n, cnt * n * 8, s_in);
+                if (irc == GL_OK && (e = hipEventRecord(ev_in[slot], s_in)) != hipSuccess)
+                    irc = hip_error(e, "pipelined batch: event");
+                if (irc == GL_OK) {
+                    std::lock_guard<std::mutex> lk(st.mu);
+                    st.ready = i + 1;
+                    st.cv.notify_all();
+                }
+            }
+        } catch (...) {
+            irc = set_error(GL_E_NOMEM, "pipelined batch: host copy failed");
+        }
+        if (irc != GL_OK) {
+            std::lock_guard<std::mutex> lk(st.mu);
+            st.in_rc = irc;
+            st.in_msg = last_error_cstr();
+            st.abort = true;
+            st.cv.notify_all();
+        }
+    };
+    std::thread helper;
+    try {
+        helper = std::thread(copy_in);
+    } catch (...) {
+        return kPipeUnavailable;  // no thread: the serial path
+    }
+    for (size_t i = 0; i < nchunks && rc == GL_OK; i++) {
+        const int slot = (int)(i & 1);
+        {
+            std::unique_lock<std::mutex> lk(st.mu);
+            st.cv.wait(lk, [&] { return st.ready > i || st.abort; });
+            if (st.ready <= i) break;  // the copy in failed
+        }
+        const size_t cnt = std::min(per, count - i * per);
+        hipError_t e = hipStreamWaitEvent(s, ev_in[slot], 0);
+        if (e != hipSuccess) {
+            rc = hip_error(e, "pipelined batch: wait");
+            break;
+        }
+        if ((rc = ntt_dev_impl(buf[slot], n, 1, cnt, inverse, s))) break;
+        e = hipEventRecord(ev_done[slot], s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s_out, ev_done[slot], 0);
+        if (e != hipSuccess) {
+            rc = hip_error(e, "pipelined batch: event");
+            break;
+        }
+        if ((rc = copy_d2h(v + i * per * n, buf[slot], cnt * n * 8, s_out, true))) break;
+        std::lock_guard<std::mutex> lk(st.mu);
+        st.slot_free[slot] = true;
+        st.cv.notify_all();
+    }
+    {
+        std::lock_guard<std::mutex> lk(st.mu);
+        st.abort = true;  // the helper stops waiting for a free buffer
+        st.cv.notify_all();
+    }
+    helper.join();
+    if (rc == GL_OK && st.in_rc != GL_OK) rc = set_error(st.in_rc, st.in_msg);
+    if (rc != GL_OK) {
+        // nothing may still read or write the buffers when they return to the pool
+        (void)hipStreamSynchronize(s_in);
+        (void)hipStreamSynchronize(s_out);
+        (void)hipStreamSynchronize(s);
+        (void)hipGetLastError();
+    }
+    return rc;
+}
+
 }  // namespace gl
 
 extern "C" {
@@ -253,7 +388,9 @@ uint64_t gl_next_size(uint64_t n) {
 
 // ---------------------------------------------------------------- NTT
 
-void gl_debug_fail_allocs(int count) { set_fail_allocs(count); }
+void gl_debug_fail_allocs(int count) { set_fail_allocs(0, count); }
+
+void gl_debug_fail_allocs_after(int skip, int count) { set_fail_allocs(skip, count); }
 
 int gl_ntt_dev(uint64_t *d, size_t n, size_t count, int inverse, void *stream) {
     return ntt_dev_impl(d, n, 1, count, inverse != 0, as_stream(stream));
@@ -270,6 +407,11 @@ int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
     if (!gl_is_divisor(n)) return get_plan(n, nullptr);
     if (count == 0) return GL_OK;
     if (!v) return set_error(GL_E_ARG, "null pointer");
+    if (words_overflow(n, count)) return set_error(GL_E_ARG, "n * count words overflow the address space");
+    if (count >= 2 && n * count * 8 >= 2 * kPipeChunk) {
+        int prc = ntt_batch_pipelined(v, n, count, inverse != 0);
+        if (prc != kPipeUnavailable) return prc;
+    }
     hipStream_t s = thread_stream();
     size_t bytes = n * count * 8;
     DevBuf d(bytes, s);

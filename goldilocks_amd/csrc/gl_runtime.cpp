@@ -107,15 +107,34 @@ void recycle_event(hipEvent_t e) {
 }
 }  // namespace
 
-// Allocations still to fail on purpose (gl_debug_fail_allocs): a host-side
-// switch for tests of the error paths; no device work is involved.
-static std::atomic<int> g_fail_allocs{0};
+// Allocations to fail on purpose (gl_debug_fail_allocs[_after]): after
+// `skip` more allocations succeed, the next `count` fail.  A host-side switch
+// for tests of the error paths; no device work is involved.
+static std::mutex g_fail_mu;
+static std::atomic<bool> g_fail_active{false};
+static int g_fail_skip = 0, g_fail_count = 0;
 
-void set_fail_allocs(int count) { g_fail_allocs.store(count > 0 ? count : 0); }
+void set_fail_allocs(int skip, int count) {
+    std::lock_guard<std::mutex> lk(g_fail_mu);
+    g_fail_skip = skip > 0 ? skip : 0;
+    g_fail_count = count > 0 ? count : 0;
+    g_fail_active.store(g_fail_count > 0);
+}
+
+static bool fail_this_alloc() {
+    if (!g_fail_active.load()) return false;
+    std::lock_guard<std::mutex> lk(g_fail_mu);
+    if (g_fail_count == 0) return false;
+    if (g_fail_skip > 0) {
+        g_fail_skip--;
+        return false;
+    }
+    if (--g_fail_count == 0) g_fail_active.store(false);
+    return true;
+}
 
 void *pool_alloc(size_t bytes, hipStream_t s) {
-    for (int f = g_fail_allocs.load(); f > 0;)
-        if (g_fail_allocs.compare_exchange_weak(f, f - 1)) return nullptr;
+    if (fail_this_alloc()) return nullptr;
     const size_t sz = round_size(bytes);
     {
         std::lock_guard<std::mutex> lk(g_pool_mu);
@@ -214,6 +233,19 @@ hipStream_t thread_stream() {
         }
     }
     return t_stream.s;
+}
+
+static thread_local ThreadStream t_aux[2];
+
+hipStream_t thread_aux_stream(int i) {
+    ThreadStream &t = t_aux[i & 1];
+    if (!t.s) {
+        if (hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            t.s = nullptr;
+        }
+    }
+    return t.s;
 }
 
 // ---------------------------------------------------------------- twiddles

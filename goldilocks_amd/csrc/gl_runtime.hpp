@@ -47,7 +47,7 @@ void pool_free(void *p, hipStream_t s);
 // Free every idle block whose last use has completed (no device sync).
 void pool_release_idle();
 // The next `count` pool_alloc calls return nullptr (error-path tests).
-void set_fail_allocs(int count);
+void set_fail_allocs(int skip, int count);
 
 // A pool block owned for the duration of one enqueue on a stream.
 struct DevBuf {
@@ -62,6 +62,9 @@ struct DevBuf {
 };
 
 hipStream_t thread_stream();
+// Two more per-thread streams (i = 0, 1) for copies that overlap the
+// thread's compute stream (the pipelined host batch).
+hipStream_t thread_aux_stream(int i);
 
 // ---- host <-> device copies of the host-buffer entry points ----
 // Large copies are staged through pinned chunks (gl_hostio.cpp).

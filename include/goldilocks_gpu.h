@@ -173,6 +173,10 @@ int gl_timing_collect(uint32_t *variant_ids, float *ms, size_t cap, size_t *coun
  * library fail as if hipMalloc had, so callers can exercise the GL_E_NOMEM
  * paths of every entry point and of the server; 0 clears it. */
 void gl_debug_fail_allocs(int count);
+/* The same after the next `skip` allocations have succeeded (a failure in
+ * the middle of a multi-allocation call, e.g. a later chunk of a pipelined
+ * host batch). */
+void gl_debug_fail_allocs_after(int skip, int count);
 
 /* Drop cached plans and device twiddle tables (ntt/src/ntt/mod.rs:43-46).
  * Tables still referenced by work being enqueued or by a batch plan stay

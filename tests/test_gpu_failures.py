@@ -62,6 +62,32 @@ def test_host_entry_points_fail_cleanly(gl, oracle):
     assert np.array_equal(y, oracle.ntt(x))
 
 
+def test_pipelined_host_batch_fails_cleanly(gl, oracle):
+    """A host batch large enough for the pipelined path (nine 2^20-word
+    transforms, chunks of 4 + 4 + 1 over two device buffers): an allocation
+    failure before the first chunk leaves the input untouched; one in the
+    middle (the second chunk's transform scratch, after the two buffers and
+    the first chunk's scratch) returns GL_E_NOMEM with the helper thread
+    joined; the library works afterwards."""
+    n, count = 1 << 20, 9
+    x = oracle.fill_random(n * count, 64)
+    s = gl.strategy(n)
+    s.ntt(x.copy())  # plan and tables built
+    for skip in (0, 3):
+        note(f"forced allocation failure: pipelined host batch, skip {skip}")
+        v = x.copy()
+        gl.lib().gl_debug_fail_allocs_after(skip, 1)
+        with pytest.raises(Exception) as e:
+            s.ntt(v)
+        assert getattr(e.value, "code", None) == GL_E_NOMEM, (skip, e.value)
+        gl.lib().gl_debug_fail_allocs(0)
+        if skip == 0:
+            assert np.array_equal(v, x), "input untouched"
+    y = x.copy()
+    s.ntt(y)
+    assert np.array_equal(y[:n], oracle.ntt(x[:n])) and np.array_equal(y[-n:], oracle.ntt(x[-n:]))
+
+
 def test_server_job_fails_and_server_continues(gl, oracle):
     n, cols = 1 << 16, 4
     x = oracle.fill_random(n * cols, 63)

@@ -104,3 +104,36 @@ def test_host_commit_and_tree_staged(gl, oracle):
     data = oracle.fill_random(3 * 2**19, 32)
     layers = gl.merkle_tree(data)
     assert layers == oracle.merkle_tree(data.tobytes())
+
+
+# (n, count): batches of at least 64 MiB run as a pipeline of 32 MiB chunks of
+# whole transforms (or one larger transform) over two device buffers; these
+# take chunks of 4 + 4 + 1, 5 + 5 + 1 and 1 + 1 + 1 transforms, 20 chunks
+# that cycle both buffers, and three 64 MiB transforms of one chunk each
+PIPELINED = [(2**20, 9), (3 * 2**18, 11), (2**22, 3), (5 * 2**12, 4000), (2**23, 3)]
+
+
+@pytest.mark.parametrize("n,count", PIPELINED)
+def test_host_batch_pipelined(gl, oracle, n, count):
+    """strategy(n).ntt on a host array of many transforms (the pipelined
+    path) against the device path on the same words, forward and inverse;
+    the first and last transforms also against the oracle."""
+    import torch
+    note(f"pipelined host batch n={n} count={count}")
+    x = oracle.fill_random(n * count, 91 + n)
+    for inverse in (False, True):
+        y = x.copy()
+        gl.strategy(n).ntt(y, inverse=inverse)
+        t = to_torch(x)
+        gl.strategy(n).ntt(t, inverse=inverse)
+        torch.cuda.synchronize()
+        assert np.array_equal(y, from_torch(t)), (n, count, inverse)
+        if n <= 1 << 20:
+            for j in (0, count - 1):
+                want = (oracle.intt if inverse else oracle.ntt)(x[j * n:(j + 1) * n].copy())
+                assert np.array_equal(y[j * n:(j + 1) * n], want), (n, count, inverse, j)
+    y = x.copy()
+    s = gl.strategy(n)
+    s.ntt(y)
+    s.ntt(y, inverse=True)
+    assert np.array_equal(y, x), (n, count)
