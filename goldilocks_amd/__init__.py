@@ -169,6 +169,20 @@ def _dev_ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _dev_ptr_arrays(tensors):
+    """Device addresses and lengths of a list of GPU tensors as uint64
+    arrays, validating each (contiguous, 64-bit, on the GPU) in one pass:
+    the batch entry points take thousands of small tensors, where per-tensor
+    ctypes objects cost more than the transforms."""
+    for t in tensors:
+        if not (t.is_cuda and t.element_size() == 8 and t.is_contiguous()):
+            raise TypeError("expected contiguous 64-bit torch tensors on the GPU")
+    k = len(tensors)
+    ptrs = np.fromiter((t.data_ptr() for t in tensors), dtype=np.uint64, count=k)
+    ns = np.fromiter((t.numel() for t in tensors), dtype=np.uint64, count=k)
+    return ptrs, ns
+
+
 def _bytes_dev_ptr(t):
     """Device address of a contiguous GPU tensor of any dtype."""
     if not t.is_cuda or not t.is_contiguous():
@@ -329,12 +343,12 @@ def ntt_batch_mixed(arrays, inverse=False, batched=True):
     k = len(arrays)
     if k == 0:
         return arrays
-    ns = (ctypes.c_size_t * k)(*[_len(a) for a in arrays])
     if _is_torch(arrays[0]):
-        ptrs = (ctypes.c_void_p * k)(*[_dev_ptr(a).value for a in arrays])
+        ptrs, ns = _dev_ptr_arrays(arrays)
         fn = lib().gl_ntt_mixed_dev if batched else lib().gl_ntt_mixed_dev_unbatched
-        _check(fn(ptrs, ns, k, int(inverse), _stream_ptr(arrays[0])))
+        _check(fn(ptrs.ctypes.data, ns.ctypes.data, k, int(inverse), _stream_ptr(arrays[0])))
     else:
+        ns = (ctypes.c_size_t * k)(*[_len(a) for a in arrays])
         ptrs = (ctypes.c_void_p * k)(*[_host_u64(a).ctypes.data for a in arrays])
         _check(lib().gl_ntt_batch_mixed(ptrs, ns, k, int(inverse)))
     return arrays
@@ -605,10 +619,9 @@ class BatchPlan:
     def __init__(self, tensors, inverse=False):
         k = len(tensors)
         self._tensors = list(tensors)  # keep the buffers alive
-        ns = (ctypes.c_size_t * k)(*[_len(t) for t in tensors])
-        ptrs = (ctypes.c_void_p * k)(*[_dev_ptr(t).value for t in tensors])
+        ptrs, ns = _dev_ptr_arrays(self._tensors)
         h = ctypes.c_void_p()
-        _check(lib().gl_batch_plan_create(ptrs, ns, k, int(inverse), ctypes.byref(h)))
+        _check(lib().gl_batch_plan_create(ptrs.ctypes.data, ns.ctypes.data, k, int(inverse), ctypes.byref(h)))
         self._h = h
 
     def launch(self, stream=None):

@@ -15,6 +15,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -97,7 +98,12 @@ int plan_batch(uint64_t *const *d, const size_t *n, size_t k, bool inverse, Sche
     return GL_OK;
 }
 
-int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Schedule &sc) {
+// Called after the groups of each (chunk, pass level) have been appended:
+// groups [g0, end), their descriptors [d0, end) and block starts [s0, end).
+using LevelFn = std::function<int(size_t g0, size_t d0, size_t s0)>;
+
+int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Schedule &sc,
+                 const LevelFn &on_level = nullptr) {
     int rc;
     for (const std::vector<size_t> &chunk : sc.chunks) {
         size_t levels = 0;
@@ -110,6 +116,7 @@ int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Sc
             cur.push_back(c);
         }
         for (size_t L = 0; L < levels; L++) {
+            const size_t g0 = sc.groups.size(), d0 = sc.descs.size(), s0 = sc.starts.size();
             std::vector<std::pair<const PassVariant *, size_t>> items;  // (variant, position in chunk)
             for (size_t j = 0; j < chunk.size(); j++)
                 if (L < sc.plans[chunk[j]]->passes.size()) items.emplace_back(sc.plans[chunk[j]]->passes[L].v, j);
@@ -136,6 +143,7 @@ int build_groups(uint64_t *const *d, const size_t *n, bool inverse, u64 *tmp, Sc
                 sc.groups.push_back(g);
                 a = b;
             }
+            if (on_level && sc.groups.size() > g0 && (rc = on_level(g0, d0, s0))) return rc;
         }
         for (PassCursor &c : cur) sc.keep.insert(sc.keep.end(), c.keep.begin(), c.keep.end());
     }
@@ -189,17 +197,46 @@ extern "C" int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, i
         if ((rc = gl_ntt_dev(d[i], n[i], 1, inverse, stream))) return rc;
     DevBuf tmp(sc.tmp_words * 8, s);
     if (sc.tmp_words && !tmp.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
-    if ((rc = build_groups(d, n, inverse != 0, tmp.as<u64>(), sc))) return rc;
-    if (sc.groups.empty()) return GL_OK;
-    DevBuf meta(sc.meta_bytes() + 512, s);
+    // Each (chunk, pass level) is uploaded and launched as soon as its
+    // descriptors are built, so the device runs one level while the host
+    // plans the next.  Level blocks [PassArgs x nd][u32 block starts],
+    // 256-byte aligned, go through a pinned slab: at most one descriptor per
+    // pass of every grouped transform, and one block-start terminator per
+    // group (at most one group per descriptor).
+    size_t dmax = 0;
+    for (const std::vector<size_t> &chunk : sc.chunks)
+        for (size_t i : chunk) dmax += sc.plans[i]->passes.size();
+    if (dmax == 0) return GL_OK;
+    const size_t cap = dmax * (sizeof(PassArgs) + 256) + 2 * dmax * sizeof(u32);
+    DevBuf meta(cap, s);
     if (!meta.p) return set_error(GL_E_NOMEM, "descriptor allocation failed");
-    std::vector<unsigned char> host;
-    pack_meta(sc, host);
-    GL_HIP(hipMemcpyAsync(meta.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
-    if ((rc = launch_groups(sc, meta.p, s, true))) return rc;
-    // `host` is released on return: wait until the copy has consumed it.
-    GL_HIP(hipStreamSynchronize(s));
-    return GL_OK;
+    PinnedSlab pin;
+    if ((rc = pinned_acquire(cap, pin))) return rc;
+    size_t off = 0;
+    auto upload_and_launch = [&](size_t g0, size_t d0, size_t s0) -> int {
+        const size_t nd = sc.descs.size() - d0, ns = sc.starts.size() - s0;
+        const size_t dbytes = nd * sizeof(PassArgs), bytes = dbytes + ns * sizeof(u32);
+        if (off + bytes > cap) return set_error(GL_E_UNSUPPORTED, "batch descriptors exceed their buffer");
+        char *h = (char *)pin.p + off, *dv = (char *)meta.p + off;
+        std::memcpy(h, sc.descs.data() + d0, dbytes);
+        std::memcpy(h + dbytes, sc.starts.data() + s0, ns * sizeof(u32));
+        GL_HIP(hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, s));
+        const PassArgs *ddesc = (const PassArgs *)dv;
+        const u32 *dstarts = (const u32 *)(dv + dbytes);
+        for (size_t gi = g0; gi < sc.groups.size(); gi++) {
+            const Group &g = sc.groups[gi];
+            if (g.blocks == 0) continue;
+            hipEvent_t e0 = timing_begin(s);
+            g.v->launch_grouped(ddesc + (g.desc_off - d0), dstarts + (g.starts_off - s0), (u32)g.nd, g.blocks, s);
+            GL_HIP(hipGetLastError());
+            timing_end(g.v, e0, s);
+        }
+        off += (bytes + 255) / 256 * 256;
+        return GL_OK;
+    };
+    rc = build_groups(d, n, inverse != 0, tmp.as<u64>(), sc, upload_and_launch);
+    pinned_release(pin, s);  // reused only after the uploads enqueued so far
+    return rc;
 }
 
 extern "C" int gl_ntt_batch_mixed(uint64_t *const *v, const size_t *n, size_t k, int inverse) {

@@ -181,7 +181,89 @@ void par_memcpy(void *dst, const void *src, size_t n) {
     latch->cv.wait(lk, [&latch] { return latch->left == 0; });
 }
 
+// Pinned host slabs for small uploads that must stay asynchronous (launch
+// descriptors): a slab returns to the pool with an event recorded behind the
+// uploads that read it, and is handed out again only once that event has
+// completed.
+constexpr int kMaxSlabs = 8;
+std::mutex g_slab_mu;
+std::vector<PinnedSlab> g_slab_free;
+int g_slabs = 0;
+
+bool slab_ready(const PinnedSlab &sl) {
+    const hipError_t e = hipEventQuery(sl.ev);
+    if (e == hipSuccess) return true;
+    (void)hipGetLastError();  // hipErrorNotReady
+    return false;
+}
+
 }  // namespace
+
+int pinned_acquire(size_t bytes, PinnedSlab &out) {
+    out = PinnedSlab();
+    PinnedSlab old;  // a free slab taken over: reused once ready, or freed if too small
+    {
+        std::lock_guard<std::mutex> lk(g_slab_mu);
+        for (size_t i = 0; i < g_slab_free.size(); i++) {
+            if (g_slab_free[i].bytes >= bytes && slab_ready(g_slab_free[i])) {
+                out = g_slab_free[i];
+                g_slab_free.erase(g_slab_free.begin() + (long)i);
+                return GL_OK;
+            }
+        }
+        if (g_slabs < kMaxSlabs) {
+            g_slabs++;  // a new slab, allocated below
+        } else if (!g_slab_free.empty()) {
+            size_t best = 0;  // the largest free slab, still in use by uploads
+            for (size_t i = 1; i < g_slab_free.size(); i++)
+                if (g_slab_free[i].bytes > g_slab_free[best].bytes) best = i;
+            old = g_slab_free[best];
+            g_slab_free.erase(g_slab_free.begin() + (long)best);
+        } else {
+            return set_error(GL_E_NOMEM, "every pinned descriptor slab is in use");
+        }
+    }
+    if (old.p) {
+        (void)hipEventSynchronize(old.ev);  // its uploads have to finish first
+        (void)hipGetLastError();
+        if (old.bytes >= bytes) {
+            out = old;
+            return GL_OK;
+        }
+        (void)hipEventDestroy(old.ev);
+        (void)hipHostFree(old.p);
+        (void)hipGetLastError();
+    }
+    size_t sz = size_t(64) << 10;
+    while (sz < bytes) sz *= 2;
+    PinnedSlab sl;
+    sl.bytes = sz;
+    if (hipHostMalloc(&sl.p, sz, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (sl.p) (void)hipHostFree(sl.p);
+        std::lock_guard<std::mutex> lk(g_slab_mu);
+        g_slabs--;
+        return set_error(GL_E_NOMEM, "pinned descriptor slab allocation failed");
+    }
+    out = sl;
+    return GL_OK;
+}
+
+void pinned_release(PinnedSlab &slab, hipStream_t s) {
+    if (!slab.p) return;
+    if (hipEventRecord(slab.ev, s) != hipSuccess) {
+        // no event behind the uploads: wait for them before the slab is reused
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s);
+        (void)hipGetLastError();
+        (void)hipEventRecord(slab.ev, nullptr);
+        (void)hipGetLastError();
+    }
+    std::lock_guard<std::mutex> lk(g_slab_mu);
+    g_slab_free.push_back(slab);
+    slab = PinnedSlab();
+}
 
 int copy_h2d(void *d_dst, const void *h_src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return GL_OK;

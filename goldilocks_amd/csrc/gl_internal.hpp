@@ -53,9 +53,8 @@ inline void fastdiv_params(u64 d, u32 &mul, u32 &sh) {
         sh = 0;
         return;
     }
-    u32 l = 0;
-    while ((1ull << l) < d) l++;
-    unsigned __int128 num = (unsigned __int128)1 << (31 + l);
+    const u32 l = d <= 1 ? 0 : 64 - (u32)__builtin_clzll(d - 1);  // ceil(log2 d) <= 32
+    const u64 num = 1ull << (31 + l);                               // <= 2^63: no 128-bit division
     mul = (u32)((num + d - 1) / d);
     sh = 31 + l;
 }

@@ -74,6 +74,17 @@ hipStream_t thread_aux_stream(int i);
 // enqueued, and the caller synchronizes s before reading h_dst).
 int copy_h2d(void *d_dst, const void *h_src, size_t bytes, hipStream_t s);
 int copy_d2h(void *h_dst, const void *d_src, size_t bytes, hipStream_t s, bool sync = true);
+// Pinned host memory for small asynchronous uploads (at least `bytes`,
+// from a pool of at most eight slabs).  pinned_release records an event
+// on s behind the uploads enqueued from the slab; the slab is handed out
+// again only after that event has completed.
+struct PinnedSlab {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+};
+int pinned_acquire(size_t bytes, PinnedSlab &out);
+void pinned_release(PinnedSlab &slab, hipStream_t s);
 
 // ---- twiddles (device tables, Montgomery form, cached) ----
 using TableRef = std::shared_ptr<const u64>;

@@ -100,7 +100,10 @@ int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int invers
  * packed in order into chunks of at most gl_set_batch_chunk_bytes bytes
  * (default 1 GiB, or GOLDILOCKS_BATCH_CHUNK_BYTES); each chunk runs as
  * grouped launches, one per (pass level, kernel variant).  Larger transforms
- * and radix-65537 ones run on their own.  Returns after the stream has consumed the launch data. */
+ * and radix-65537 ones run on their own.  Enqueues on `stream` and returns,
+ * like the other device entry points: each (chunk, pass level) is launched
+ * as soon as its descriptors are built (uploaded through pinned memory), so
+ * the device runs one level while the host plans the next. */
 int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, int inverse, void *stream);
 
 /* Chunk size of the grouped batch path; returns the previous value.  Results

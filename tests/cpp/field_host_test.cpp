@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../goldilocks_amd/csrc/gl_dft.hpp"
+#include "../../goldilocks_amd/csrc/gl_internal.hpp"
 
 extern "C" int or_naive_ntt(uint64_t *, size_t);
 extern "C" void or_fill_random(uint64_t *, size_t, uint64_t);
@@ -96,6 +97,23 @@ int main() {
     check_codelet<8>(); check_codelet<10>(); check_codelet<15>(); check_codelet<16>(); check_codelet<17>();
     check_codelet<20>(); check_codelet<32>(); check_codelet<34>(); check_codelet<40>(); check_codelet<48>();
     check_codelet<51>(); check_codelet<60>(); check_codelet<64>(); check_codelet<85>();
-    printf("%zu field values and 19 codelet sizes: %s\n", v.size(), g_bad ? "FAIL" : "ok");
+    // the kernels' instance division: (x * mul) >> sh == x / d for x < 2^31
+    // and 1 <= d < 2^32 (fastdiv_params), near every power of two and at random
+    std::vector<u64> ds;
+    for (int k = 0; k < 32; k++)
+        for (int o = -2; o <= 2; o++)
+            if ((1ll << k) + o >= 1) ds.push_back((1ull << k) + o);
+    for (int i = 0; i < 2000; i++) ds.push_back((g() >> (g() % 33 + 31)) | 1);
+    for (u64 d : ds) {
+        u32 m, sh;
+        fastdiv_params(d, m, sh);
+        const u64 top = (u64(1) << 31) - 1;
+        for (u64 x : {u64(0), d - 1, d, d + 1, 2 * d - 1, top, top - (top + 1) % d, u64(g() % (top + 1)),
+                      u64(g() % (top + 1))}) {
+            if (x > top) continue;
+            if (fastdiv((u32)x, m, sh) != x / d) fail("fastdiv", x, d);
+        }
+    }
+    printf("%zu field values, 19 codelet sizes, %zu divisors: %s\n", v.size(), ds.size(), g_bad ? "FAIL" : "ok");
     return g_bad != 0;
 }
