@@ -10,6 +10,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <unordered_map>
 
 #include "gl_field.hpp"
 
@@ -350,13 +352,45 @@ PowTable power_table(u64 base, u64 count, bool create) {
     return t;
 }
 
-PowTable root_table(u64 M, bool create) { return power_table(root(M), M, create); }
+// root(M) memoized: planning looks up the tables of every pass of every
+// transform of a batch, and the exponentiation costs more than the lookup.
+static u64 root_cached(u64 M) {
+    static std::mutex mu;
+    static std::unordered_map<u64, u64> memo;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = memo.find(M);
+        if (it != memo.end()) return it->second;
+    }
+    const u64 r = root(M);
+    std::lock_guard<std::mutex> lk(mu);
+    memo.emplace(M, r);
+    return r;
+}
+
+PowTable root_table(u64 M, bool create) { return power_table(root_cached(M), M, create); }
+
+// Montgomery form of 1/n (the inverse transform's scale), memoized likewise.
+static u64 inv_scale_cached(u64 n) {
+    static std::mutex mu;
+    static std::unordered_map<u64, u64> memo;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = memo.find(n);
+        if (it != memo.end()) return it->second;
+    }
+    const u64 r = to_mont(inv(n % P));
+    std::lock_guard<std::mutex> lk(mu);
+    memo.emplace(n, r);
+    return r;
+}
 
 TableRef twiddle_matrix(u64 M, u64 R, bool create) {
-    if (!create) {
+    {
         std::lock_guard<std::mutex> lk(g_tw_mu);
         auto it = g_full_tables.find(std::make_pair(M, R));
-        return it != g_full_tables.end() ? it->second : nullptr;
+        if (it != g_full_tables.end()) return it->second;
+        if (!create) return nullptr;
     }
     PowTable t = root_table(M);
     if (!t.lo || !t.hi) return nullptr;
@@ -897,7 +931,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
             if (inverse && p == 0 && scale_out && !row_scale && np > 1 &&
                 (pp.v->kind == VK_TWO_STAGE || pp.v->kind == VK_THREE_STAGE)) {
                 a.flags |= FLAG_SCALE_TW;
-                a.scale = to_mont(inv(n % P));
+                a.scale = inv_scale_cached(n);
                 scale_folded = true;
             }
         }
@@ -919,7 +953,7 @@ int PassCursor::next(PassArgs &a, unsigned &grid) {
         a.row_scale = row_scale;
     } else if (inverse && last && scale_out && !scale_folded) {
         a.flags |= FLAG_SCALE;
-        a.scale = to_mont(inv(n % P));
+        a.scale = inv_scale_cached(n);
     }
     grid = (unsigned)((total + pp.v->T - 1) / pp.v->T);
     src = dst;
