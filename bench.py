@@ -331,6 +331,22 @@ def bench_ntt(args, torch, gl):
         host_s = statistics.median(host_times)
         host = {"gelem_s": round(n / host_s / 1e9, 3), "ms": round(host_s * 1e3, 3),
                 "path": "gl_ntt on a pageable host array (PCIe copies included)"}
+        # a host batch (strategy(n).ntt on many transforms): copies in,
+        # transforms and copies out overlap in chunks
+        bn, bc = 1 << 20, 64
+        hb = np.concatenate([h[:bn]] * bc)
+        s = gl.strategy(bn)
+        s.ntt(hb)
+        batch_times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s.ntt(hb)
+            batch_times.append(time.perf_counter() - t0)
+        del hb
+        bs = statistics.median(batch_times)
+        host["batch"] = {"n": bn, "count": bc, "ms": round(bs * 1e3, 3),
+                         "gelem_s": round(bn * bc / bs / 1e9, 3),
+                         "path": "strategy(n).ntt on a pageable host array of count transforms"}
     # Achievable streaming rate on this device: a plain device copy of the
     # same n words (8n bytes read, 8n written), timed like the steps.
     buf = torch.empty_like(x)
