@@ -137,3 +137,41 @@ def test_host_batch_pipelined(gl, oracle, n, count):
     s.ntt(y)
     s.ntt(y, inverse=True)
     assert np.array_equal(y, x), (n, count)
+
+
+def test_pipelined_host_batches_from_two_threads(gl, oracle):
+    """Two host threads run pipelined host batches at once (each with its
+    own streams and copy thread, sharing the staging pool): every result
+    equals the device path's."""
+    import torch
+    cases = [(2**20, 9, 1), (3 * 2**18, 11, 2)]
+    refs = {}
+    for n, count, seed in cases:
+        x = oracle.fill_random(n * count, 950 + seed)
+        t = to_torch(x)
+        gl.strategy(n).ntt(t)
+        torch.cuda.synchronize()
+        refs[seed] = (x, from_torch(t))
+    errors = []
+
+    def worker(n, count, seed):
+        try:
+            x, ref = refs[seed]
+            for _ in range(3):
+                y = x.copy()
+                gl.strategy(n).ntt(y)
+                if not np.array_equal(y, ref):
+                    errors.append(f"forward n={n} count={count}")
+                gl.strategy(n).ntt(y, inverse=True)
+                if not np.array_equal(y, x):
+                    errors.append(f"roundtrip n={n} count={count}")
+        except Exception as e:  # surfaced below
+            errors.append(repr(e))
+
+    note("pipelined host batches: two threads")
+    ts = [threading.Thread(target=worker, args=c) for c in cases]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
