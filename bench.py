@@ -8,7 +8,8 @@ sample).  Extra workloads: --workload batch (mixed-size batched NTTs,
 batched vs one-launch-sequence-per-transform, with p50/p99 latency),
 --workload commit (coset LDE + K12 Merkle commit), --workload serve (open-loop
 Poisson NTT/commit jobs through the continuous-batching server), --workload
-permute (the reference's transpose and bit_reverse on HBM-resident words).
+permute (the reference's transpose and bit_reverse on HBM-resident words),
+--workload hostbatch (NTT batches on pageable host arrays, PCIe included).
 
 The default line also carries the metric's second half, "mixed_batch": the
 1024-transform mixed-size batch (n = 2^k * f, k in [10, 20], f in
@@ -34,7 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--log-n", type=int, default=24)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve", "permute"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve", "permute", "hostbatch"])
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
     ap.add_argument("--slots", type=int, default=8, help="serve: server slots (streams), 1-16")
     ap.add_argument("--no-host-buffer", action="store_true",
@@ -441,6 +442,9 @@ def main():
         from goldilocks_amd.benchmarks import bench_permute
         out = bench_permute(args, torch, gl)
         out["cpu_baseline"] = cpu_baseline_permute(min(args.cpu_seconds, 4.0))
+    elif args.workload == "hostbatch":
+        from goldilocks_amd.benchmarks import bench_hostbatch
+        out = bench_hostbatch(args, torch, gl)
     else:
         from goldilocks_amd.benchmarks import bench_serve
         out = bench_serve(args, torch, gl)

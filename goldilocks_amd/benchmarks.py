@@ -344,3 +344,49 @@ def bench_permute(args, torch, gl):
         "config": {"workload": head["case"]}, "cases": cases,
         "copy_gbs": round(16 * r * c / (copy_ms * 1e-3) / 1e9, 1),
     }
+
+
+def bench_hostbatch(args, torch, gl):
+    """The host-slice batch path (strategy(n).ntt on a pageable numpy array
+    of count transforms, the reference's own calling convention): one call
+    per batch, whose copies in, transforms and copies out overlap in chunks,
+    against one call per transform on the same words (each a copy in, the
+    transform and a copy out, one after another).  Median of 5 wall times
+    each; rates count the words moved both ways over PCIe."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    cases = []
+    for n, count in ((1 << 20, 64), (1 << 22, 16), (5 << 12, 4000), (1 << 24, 8)):
+        x = rng.integers(0, (1 << 64) - (1 << 32), size=n * count, dtype=np.uint64)
+        s = gl.strategy(n)
+
+        def timed(fn):
+            fn()
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return sorted(ts)[2]
+
+        def per_transform():
+            for j in range(count):
+                s.ntt(x[j * n:(j + 1) * n])
+
+        tb = timed(lambda: s.ntt(x))
+        tp = timed(per_transform)
+        cases.append({"n": n, "count": count, "mib": round(n * count * 8 / 2**20),
+                      "batch_ms": round(tb * 1e3, 3), "batch_gelem_s": round(n * count / tb / 1e9, 3),
+                      "batch_pcie_gbs": round(16 * n * count / tb / 1e9, 1),
+                      "per_transform_ms": round(tp * 1e3, 3),
+                      "per_transform_gelem_s": round(n * count / tp / 1e9, 3),
+                      "speedup": round(tp / tb, 3)})
+        del x
+    head = cases[0]
+    return {
+        "metric": "Gelem/s of NTT batches on pageable host arrays (PCIe copies included)",
+        "value": head["batch_gelem_s"], "unit": "Gelem/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "u64", "data": "synthetic: seeded uniform [0,p) words in host memory",
+        "config": {"workload": f"strategy({head['n']}).ntt on {head['count']} transforms in one host array"},
+        "cases": cases,
+    }

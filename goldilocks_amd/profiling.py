@@ -58,6 +58,54 @@ def kernel_stats(path):
     return sorted(rows, key=lambda x: -x[2])
 
 
+def _union(intervals):
+    """Sorted, merged [start, end) intervals."""
+    out = []
+    for a, b in sorted(intervals):
+        if out and a <= out[-1][1]:
+            out[-1][1] = max(out[-1][1], b)
+        else:
+            out.append([a, b])
+    return out
+
+
+def _overlap(a, b):
+    """Total length of the intersection of two merged interval lists."""
+    i = j = 0
+    both = 0
+    while i < len(a) and j < len(b):
+        lo, hi = max(a[i][0], b[j][0]), min(a[i][1], b[j][1])
+        if lo < hi:
+            both += hi - lo
+        if a[i][1] < b[j][1]:
+            i += 1
+        else:
+            j += 1
+    return both
+
+
+def copy_activity(copies_csv, kernels_csv=None):
+    """Host <-> device copy activity per stream: DMA copies from a rocprofv3
+    --memory-copy-trace CSV (with their direction) and, from a --kernel-trace
+    CSV, the runtime's blit kernels (__amd_rocclr_copyBuffer), which carry
+    some copies instead of the DMA engines.  Returns {stream: {"dma": {dir:
+    n}, "blits": n, "busy": merged intervals}}."""
+    acts = collections.defaultdict(lambda: {"dma": collections.Counter(), "blits": 0, "iv": []})
+    with open(copies_csv) as f:
+        for r in csv.DictReader(f):
+            a = acts[r["Stream_Id"]]
+            a["dma"][r["Direction"].replace("MEMORY_COPY_", "")] += 1
+            a["iv"].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    if kernels_csv:
+        with open(kernels_csv) as f:
+            for r in csv.DictReader(f):
+                if "copyBuffer" in r["Kernel_Name"]:
+                    a = acts[r["Stream_Id"]]
+                    a["blits"] += 1
+                    a["iv"].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    return {s: {"dma": dict(a["dma"]), "blits": a["blits"], "busy": _union(a["iv"])} for s, a in acts.items()}
+
+
 def write_stats_md(f, args, stats):
     f.write(f"# rocprofv3 summary: {args.workload}\n\n")
     if args.command:
@@ -112,7 +160,33 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--counters", default=None, help="counter_collection.csv of a --pmc run")
     ap.add_argument("--kernel", default="ntt_pass", help="kernel-name filter for --counters")
+    ap.add_argument("--copies", default=None, help="memory_copy_trace.csv of a --memory-copy-trace run")
+    ap.add_argument("--kernel-trace", default=None, help="kernel_trace.csv of the same run (blit copies)")
     args = ap.parse_args()
+    if args.copies:
+        acts = copy_activity(args.copies, args.kernel_trace)
+        with open(args.md, "w") as f:
+            if args.stats:
+                write_stats_md(f, args, kernel_stats(args.stats))
+            f.write("\nHost <-> device copies per stream (DMA copies from `--memory-copy-trace`, "
+                    "blit kernels from `--kernel-trace`):\n\n")
+            f.write("| stream | DMA copies | blit kernels | copy time in flight ms |\n|---|---|---|---|\n")
+            for st, a in sorted(acts.items(), key=lambda x: int(x[0])):
+                dma = ", ".join(f"{n} {d}" for d, n in sorted(a["dma"].items())) or "-"
+                busy = sum(b - a_ for a_, b in a["busy"]) / 1e6
+                f.write(f"| {st} | {dma} | {a['blits']} | {busy:.3f} |\n")
+            pairs = []
+            keys = sorted(acts, key=int)
+            for x in range(len(keys)):
+                for y in range(x + 1, len(keys)):
+                    both = _overlap(acts[keys[x]]["busy"], acts[keys[y]]["busy"])
+                    if both:
+                        pairs.append((keys[x], keys[y], both))
+            if pairs:
+                f.write("\nCopies of two streams in flight at once:\n\n| streams | ms |\n|---|---|\n")
+                for a_, b, both in pairs:
+                    f.write(f"| {a_} and {b} | {both / 1e6:.3f} |\n")
+        return
     if args.counters:
         with open(args.md, "w") as f:
             f.write(f"# rocprofv3 counters: {args.workload}\n\n")
