@@ -1,6 +1,6 @@
 """GPU parity at the workloads' own sizes, through the C ABI.
 
-- 2^24 and 2^27 forward transforms compared with the oracle on every word
+- 2^24, 2^27 and 2^28 forward transforms compared with the oracle on every word
   (the contract of ntt/src/ntt/mod.rs:128-139: every output equals the exact
   DFT), then the exact roundtrip;
 - every schedule of the shipped tune file (goldilocks_amd/tune_gfx950.txt),
@@ -67,10 +67,10 @@ def _sum_mod_p(t):
     return total % P
 
 
-@pytest.mark.parametrize("k", [24, 27])
+@pytest.mark.parametrize("k", [24, 27, 28])
 def test_full_elementwise(gl, oracle, k):
-    """Every output word of the 2^24 / 2^27 forward transform equals the
-    oracle's; the inverse restores the input exactly."""
+    """Every output word of the 2^24 / 2^27 / 2^28 forward transform (three
+    schedules) equals the oracle's; the inverse restores the input exactly."""
     import torch
     n = 1 << k
     note(f"full element-wise n=2^{k}")
