@@ -1,6 +1,7 @@
 """GPU parity at the workloads' own sizes, through the C ABI.
 
-- 2^24, 2^27 and 2^28 forward transforms compared with the oracle on every word
+- 2^24, 2^27, 2^28 and the mixed batch's largest lengths (257 * 2^20,
+  15 * 2^22) forward, compared with the oracle on every word
   (the contract of ntt/src/ntt/mod.rs:128-139: every output equals the exact
   DFT), then the exact roundtrip;
 - every schedule of the shipped tune file (goldilocks_amd/tune_gfx950.txt),
@@ -67,15 +68,19 @@ def _sum_mod_p(t):
     return total % P
 
 
-@pytest.mark.parametrize("k", [24, 27, 28])
-def test_full_elementwise(gl, oracle, k):
-    """Every output word of the 2^24 / 2^27 / 2^28 forward transform (three
-    schedules) equals the oracle's; the inverse restores the input exactly."""
+FULL = [(1 << 24, "2^24"), (1 << 27, "2^27"), (1 << 28, "2^28"), (257 << 20, "257*2^20"), (15 << 22, "15*2^22")]
+
+
+@pytest.mark.parametrize("n", [n for n, _ in FULL], ids=[i for _, i in FULL])
+def test_full_elementwise(gl, oracle, n):
+    """Every output word of the forward transform equals the oracle's at
+    2^24, 2^27 and 2^28 (three schedules) and at the mixed batch's largest
+    lengths, 257 * 2^20 (two 1024-point passes and the radix-257 Rader pass)
+    and 15 * 2^22; the inverse restores the input exactly."""
     import torch
-    n = 1 << k
-    note(f"full element-wise n=2^{k}")
+    note(f"full element-wise n={n}")
     x = torch.empty(n, dtype=torch.int64, device="cuda")
-    gl.fill_random(x, 2000 + k)
+    gl.fill_random(x, 2000 + n.bit_length())
     host_x = from_torch(x)
     y = x.clone()
     gl.ntt(y)
