@@ -211,16 +211,16 @@ int pinned_acquire(size_t bytes, PinnedSlab &out) {
                 return GL_OK;
             }
         }
-        if (g_slabs < kMaxSlabs) {
-            g_slabs++;  // a new slab, allocated below
-        } else if (!g_slab_free.empty()) {
+        if (g_slabs < kMaxSlabs || g_slab_free.empty()) {
+            // a new slab, allocated below (beyond kMaxSlabs only while more
+            // callers than that hold one at once)
+            g_slabs++;
+        } else {
             size_t best = 0;  // the largest free slab, still in use by uploads
             for (size_t i = 1; i < g_slab_free.size(); i++)
                 if (g_slab_free[i].bytes > g_slab_free[best].bytes) best = i;
             old = g_slab_free[best];
             g_slab_free.erase(g_slab_free.begin() + (long)best);
-        } else {
-            return set_error(GL_E_NOMEM, "every pinned descriptor slab is in use");
         }
     }
     if (old.p) {

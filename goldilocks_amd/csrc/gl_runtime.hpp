@@ -75,7 +75,8 @@ hipStream_t thread_aux_stream(int i);
 int copy_h2d(void *d_dst, const void *h_src, size_t bytes, hipStream_t s);
 int copy_d2h(void *h_dst, const void *d_src, size_t bytes, hipStream_t s, bool sync = true);
 // Pinned host memory for small asynchronous uploads (at least `bytes`,
-// from a pool of at most eight slabs).  pinned_release records an event
+// from a pool of eight slabs, more only while more callers hold one at
+// once).  pinned_release records an event
 // on s behind the uploads enqueued from the slab; the slab is handed out
 // again only after that event has completed.
 struct PinnedSlab {

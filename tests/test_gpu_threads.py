@@ -8,6 +8,8 @@ plan cache allow, ntt/src/ntt/mod.rs:23-78).
   from several threads stay bit-exact (tables a call is using are never
   freed under it);
 - a captured batch plan keeps its twiddle tables across clear_cache;
+- grouped mixed batches from more threads than the pool of pinned
+  descriptor slabs holds stay bit-exact;
 - removing an explicit schedule, or clearing the caches, puts the tune
   file's schedule back in force.
 """
@@ -174,3 +176,39 @@ def test_tuned_schedule_restored(gl):
     gl.clear_cache()
     assert gl.plan_ids(n) == tuned
     assert gl.plan_ids(ncols, cols=True) == tuned_c
+
+
+def test_mixed_batches_from_twelve_threads(gl):
+    """Twelve threads run grouped mixed batches at once, more than the pool
+    of pinned descriptor slabs holds: every batch equals its transforms run
+    one by one."""
+    import random
+    import torch
+    sizes_per_thread = []
+    for t in range(12):
+        rng = random.Random(700 + t)
+        sizes_per_thread.append([(1 << rng.randint(4, 14)) * rng.choice((1, 3, 5, 17)) for _ in range(24)])
+    refs = []
+    for t, sizes in enumerate(sizes_per_thread):
+        xs = [torch.empty(n, dtype=torch.int64, device="cuda") for n in sizes]
+        for i, x in enumerate(xs):
+            gl.fill_random(x, 1000 * t + i)
+        ys = [x.clone() for x in xs]
+        for y in ys:
+            gl.ntt(y)
+        refs.append((xs, ys))
+    torch.cuda.synchronize()
+    note("grouped mixed batches from twelve threads")
+
+    def work(t):
+        def run():
+            xs, ys = refs[t]
+            for _ in range(4):
+                bufs = [x.clone() for x in xs]
+                gl.ntt_batch_mixed(bufs)
+                torch.cuda.current_stream().synchronize()
+                for b, y in zip(bufs, ys):
+                    assert torch.equal(b, y), (t, b.numel())
+        return run
+
+    _run_threads([work(t) for t in range(12)])
