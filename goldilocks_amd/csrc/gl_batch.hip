@@ -190,6 +190,12 @@ extern "C" int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, i
     if (k == 0) return GL_OK;
     if (!d || !n) return set_error(GL_E_ARG, "null pointer");
     hipStream_t s = (hipStream_t)stream;
+    // the launch descriptors come from a pinned slab that is reused once the
+    // uploads have run: a captured graph would keep reading it later
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GL_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return set_error(GL_E_UNSUPPORTED, "gl_ntt_mixed_dev cannot be captured; use gl_batch_plan_create");
     Schedule sc;
     if ((rc = plan_batch(d, n, k, inverse != 0, sc))) return rc;
     // stand-alone transforms first (each takes its own scratch)

@@ -103,7 +103,9 @@ int gl_ntt_cols_dev(uint64_t *d, size_t n, size_t cols, size_t count, int invers
  * and radix-65537 ones run on their own.  Enqueues on `stream` and returns,
  * like the other device entry points: each (chunk, pass level) is launched
  * as soon as its descriptors are built (uploaded through pinned memory), so
- * the device runs one level while the host plans the next. */
+ * the device runs one level while the host plans the next.  Not for graph
+ * capture (GL_E_UNSUPPORTED on a capturing stream): gl_batch_plan_create
+ * captures a batch. */
 int gl_ntt_mixed_dev(uint64_t *const *d, const size_t *n, size_t k, int inverse, void *stream);
 
 /* Chunk size of the grouped batch path; returns the previous value.  Results

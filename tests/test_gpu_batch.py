@@ -117,3 +117,33 @@ def test_chunked_batches_and_captured_plan(gl, oracle, chunk_bytes):
             plan.close()
     finally:
         gl.set_batch_chunk_bytes(prev)
+
+
+def test_mixed_batch_refuses_graph_capture(gl):
+    """gl_ntt_mixed_dev on a capturing stream returns GL_E_UNSUPPORTED (its
+    launch descriptors live in a reused pinned slab; gl_batch_plan_create is
+    the capturable form) and the batch still runs afterwards."""
+    import torch
+    note("mixed batch under graph capture")
+    sizes = [1 << 10, 3 << 8, 17 << 6]
+    bufs = [torch.empty(n, dtype=torch.int64, device="cuda") for n in sizes]
+    for i, b in enumerate(bufs):
+        gl.fill_random(b, 40 + i)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        g.capture_begin()
+        try:
+            with pytest.raises(Exception) as e:
+                gl.ntt_batch_mixed(bufs)
+        finally:
+            g.capture_end()
+    assert getattr(e.value, "code", None) == 5, e.value  # GL_E_UNSUPPORTED
+    want = [b.clone() for b in bufs]
+    for w in want:
+        gl.ntt(w)
+    gl.ntt_batch_mixed(bufs)
+    torch.cuda.synchronize()
+    for b, w in zip(bufs, want):
+        assert torch.equal(b, w)
