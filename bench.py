@@ -44,6 +44,8 @@ def parse():
                     help="batch: transform lengths 2^k * f with k uniform in [LO, HI]")
     ap.add_argument("--no-batch", action="store_true",
                     help="ntt: leave out the mixed-size batch sub-object")
+    ap.add_argument("--no-other-sizes", action="store_true",
+                    help="ntt: leave out the other lengths (2^16, 2^20, 2^27), e.g. under a profiler")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
     return ap.parse_args()
@@ -423,7 +425,8 @@ def main():
         out = bench_ntt(args, torch, gl)
         torch.cuda.empty_cache()
         # the plain-NTT workloads' other lengths (2^16, 2^20, 2^27)
-        out["other_sizes"] = other_sizes(torch, gl)
+        if not args.no_other_sizes:
+            out["other_sizes"] = other_sizes(torch, gl)
         if not args.no_batch:
             # the metric's second half: p50/p99 of the mixed-size batch
             from goldilocks_amd.benchmarks import batch_summary
