@@ -72,12 +72,16 @@ int ensure_init() {
 // there at the free.  A block is handed out again at once for the same
 // stream (stream order runs the earlier user first), to any other stream
 // once its event has completed.  Blocks freed on a capturing stream become
-// part of a graph and are never reused.
+// part of a graph and are never reused.  On the null stream, which is never
+// destroyed, the event is recorded only when another stream asks for the
+// block (it then marks all null-stream work enqueued so far, the block's
+// last use included): back-to-back calls on the null stream record none.
 namespace {
 struct FreeBlock {
     void *p;
     hipStream_t stream;
-    hipEvent_t ev;  // nullptr: no pending use
+    hipEvent_t ev;  // nullptr and !pending: no pending use
+    bool pending;   // last used on the null stream; no event recorded yet
 };
 std::mutex g_pool_mu;
 std::multimap<size_t, FreeBlock> g_pool_free;
@@ -95,17 +99,47 @@ size_t round_size(size_t b) {
 bool same_stream(hipStream_t a, hipStream_t b) { return a == b && a != hipStreamPerThread; }
 
 // caller holds g_pool_mu
-bool block_ready(const FreeBlock &fb, hipStream_t s) {
-    if (!fb.ev || same_stream(fb.stream, s)) return true;
-    hipError_t q = hipEventQuery(fb.ev);
-    if (q == hipSuccess) return true;
-    if (q != hipErrorNotReady) (void)hipGetLastError();
-    return false;
+hipEvent_t take_event() {
+    hipEvent_t e = nullptr;
+    if (!g_pool_events.empty()) {
+        e = g_pool_events.back();
+        g_pool_events.pop_back();
+    } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+    }
+    return e;
 }
 
 // caller holds g_pool_mu
 void recycle_event(hipEvent_t e) {
     if (e) g_pool_events.push_back(e);
+}
+
+// caller holds g_pool_mu.  A pending block gets its event now; without one
+// (no event could be made) the null stream is synchronized instead.
+void settle_pending(FreeBlock &fb) {
+    if (!fb.pending) return;
+    fb.pending = false;
+    hipEvent_t e = take_event();
+    if (e && hipEventRecord(e, nullptr) == hipSuccess) {
+        fb.ev = e;
+        return;
+    }
+    (void)hipGetLastError();
+    if (e) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(nullptr);
+}
+
+// caller holds g_pool_mu
+bool block_ready(FreeBlock &fb, hipStream_t s) {
+    if (same_stream(fb.stream, s)) return true;
+    settle_pending(fb);
+    if (!fb.ev) return true;
+    hipError_t q = hipEventQuery(fb.ev);
+    if (q == hipSuccess) return true;
+    if (q != hipErrorNotReady) (void)hipGetLastError();
+    return false;
 }
 }  // namespace
 
@@ -143,8 +177,9 @@ void *pool_alloc(size_t bytes, hipStream_t s) {
         auto lo = g_pool_free.lower_bound(sz), hi = g_pool_free.upper_bound(sz + sz / 4);
         for (int pass = 0; pass < 2; pass++) {
             for (auto it = lo; it != hi; ++it) {
-                const bool ok = pass == 0 ? (!it->second.ev || same_stream(it->second.stream, s))
-                                          : block_ready(it->second, s);
+                FreeBlock &fb = it->second;
+                const bool ok = pass == 0 ? ((!fb.ev && !fb.pending) || same_stream(fb.stream, s))
+                                          : block_ready(fb, s);
                 if (!ok) continue;
                 void *p = it->second.p;
                 recycle_event(it->second.ev);
@@ -178,22 +213,19 @@ void pool_free(void *p, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     auto it = g_pool_size.find(p);
     if (it == g_pool_size.end()) return;
-    hipEvent_t e = nullptr;
-    if (!g_pool_events.empty()) {
-        e = g_pool_events.back();
-        g_pool_events.pop_back();
-    } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-        (void)hipGetLastError();
-        e = nullptr;
+    if (s == nullptr) {
+        g_pool_free.emplace(it->second, FreeBlock{p, s, nullptr, true});
+        return;
     }
+    hipEvent_t e = take_event();
     if (e && hipEventRecord(e, s) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipEventDestroy(e);
         e = nullptr;
-        // no event: only safe on the same stream; wait for it instead
-        (void)hipStreamSynchronize(s);
     }
-    g_pool_free.emplace(it->second, FreeBlock{p, s, e});
+    // no event: only safe on the same stream; wait for it instead
+    if (!e) (void)hipStreamSynchronize(s);
+    g_pool_free.emplace(it->second, FreeBlock{p, s, e, false});
 }
 
 void pool_release_idle() {
@@ -201,6 +233,10 @@ void pool_release_idle() {
     for (auto it = g_pool_free.begin(); it != g_pool_free.end();) {
         FreeBlock &fb = it->second;
         bool done = true;
+        if (fb.pending) {
+            settle_pending(fb);
+            if (fb.ev) (void)hipEventSynchronize(fb.ev);  // its null-stream work first
+        }
         if (fb.ev) {
             hipError_t q = hipEventQuery(fb.ev);
             if (q != hipSuccess) {
