@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--n", type=int, default=0,
+                    help="ntt: transform length, any divisor of p - 1 (overrides --log-n)")
     ap.add_argument("--workload", default="ntt", choices=["ntt", "batch", "commit", "serve", "permute", "hostbatch"])
     ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 60%% of capacity)")
     ap.add_argument("--slots", type=int, default=8, help="serve: server slots (streams), 1-16")
@@ -267,8 +269,15 @@ def other_sizes(torch, gl, log_sizes=(16, 20, 27), seconds=0.3):
     return out
 
 
+def size_label(n):
+    """2^k, or f*2^k for a length with an odd factor f."""
+    k = (n & -n).bit_length() - 1
+    return f"2^{k}" if n == 1 << k else f"{n >> k}*2^{k}"
+
+
 def bench_ntt(args, torch, gl):
-    n = 1 << args.log_n
+    n = args.n or 1 << args.log_n
+    label = size_label(n)
     x = torch.empty(n, dtype=torch.int64, device="cuda")
     gl.fill_random(x, 1)
     # A plain copy of the same n words (8n bytes read, 8n written): the
@@ -378,7 +387,7 @@ def bench_ntt(args, torch, gl):
     moved = 16.0 * n / (avg_ms * 1e-3) / 1e9
     gelem = n / (ms * 1e-3) / 1e9
     return {
-        "metric": f"Gelem/s, 2^{args.log_n}-point forward Goldilocks NTT (HBM-resident)",
+        "metric": f"Gelem/s, {label}-point forward Goldilocks NTT (HBM-resident)",
         "value": round(gelem, 3),
         "unit": "Gelem/s",
         "n_gpus": 1,
@@ -391,7 +400,7 @@ def bench_ntt(args, torch, gl):
         "vs_baseline_ref": "M1 Max CPU ntt-par at n~2^24, ~1.4e8 elem/s (bench/analysis.ipynb plot)",
         "dtype": "u64",
         "data": "synthetic: seeded splitmix64 words uniform in [0,p)",
-        "config": {"workload": f"forward NTT, n=2^{args.log_n}, in place, words resident in HBM",
+        "config": {"workload": f"forward NTT, n={label}, in place, words resident in HBM",
                    "n": n, "passes": [p["kernel"] for p in plan]},
         "latency_ms": {"p50": round(p50, 5), "p99": round(p99, 5)},
         "inverse": {"ms_per_step": round(inv_ms, 5), "gelem_s": round(n / (inv_ms * 1e-3) / 1e9, 3)},
@@ -400,7 +409,7 @@ def bench_ntt(args, torch, gl):
         "host_buffer": host,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_for(kname, f"ntt_2^{args.log_n}"),
+                     "traffic": traffic_for(kname, f"ntt_{label}"),
                      "avg_launch_us": round(avg_ms * 1e3, 2), "launches_per_transform": npass,
                      "algorithmic_bytes_per_launch": int(alg_per_launch),
                      "launch_bytes_moved": 16 * n, "launch_hbm_gbs": round(moved, 1),
@@ -432,7 +441,7 @@ def main():
             from goldilocks_amd.benchmarks import batch_summary
             out["mixed_batch"] = batch_summary(torch, gl, args.batch_k)
             torch.cuda.empty_cache()
-        out["cpu_baseline"] = cpu_baseline_ntt(1 << args.log_n, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline_ntt(args.n or 1 << args.log_n, args.cpu_seconds)
     elif args.workload == "batch":
         from goldilocks_amd.benchmarks import bench_batch
         out = bench_batch(args, torch, gl)
