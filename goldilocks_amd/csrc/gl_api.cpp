@@ -87,7 +87,9 @@ int row_tree_dev_impl(const u64 *d_rows, size_t nrows, size_t cols, uint8_t *d_o
     return GL_OK;
 }
 
-int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) {
+// scratch: where the ping-pong buffer goes when the caller frees it (e.g. as
+// idle after a synchronous copy); by default the call's own, stream ordered.
+int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s, DevBuf *scratch = nullptr) {
     int rc = ensure_init();
     if (rc) return rc;
     if (inverse && n <= 1) return GL_OK;  // mod.rs:105-107
@@ -98,7 +100,9 @@ int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s) 
     if (words_overflow(n, S0, count)) return set_error(GL_E_ARG, "n * count words overflow the address space");
     const u64 bstride = n * S0;
     const size_t words = plan_tmp_words(*plan, count, bstride);
-    DevBuf tmp(words * 8, s);  // this call's scratch, stream ordered
+    DevBuf own;
+    DevBuf &tmp = scratch ? *scratch : own;
+    tmp.alloc(words * 8, s);  // this call's scratch
     if (words && !tmp.p) return set_error(GL_E_NOMEM, "scratch allocation failed");
     return run_plan(*plan, d, tmp.as<u64>(), S0, count, bstride, inverse, s);
 }
@@ -414,11 +418,16 @@ int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
     }
     hipStream_t s = thread_stream();
     size_t bytes = n * count * 8;
-    DevBuf d(bytes, s);
+    DevBuf d(bytes, s), tmp;
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     if ((rc = copy_h2d(d.p, v, bytes, s))) return rc;
-    if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s))) return rc;
-    return copy_d2h(v, d.p, bytes, s);
+    if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s, &tmp))) return rc;
+    if ((rc = copy_d2h(v, d.p, bytes, s))) return rc;
+    // the synchronous copy back ran after every use of both blocks on s:
+    // they return to the pool idle (no event to record)
+    d.release_idle();
+    tmp.release_idle();
+    return GL_OK;
 }
 
 int gl_ntt(uint64_t *v, size_t n) { return gl_ntt_batch(v, n, 1, 0); }

@@ -228,6 +228,14 @@ void pool_free(void *p, hipStream_t s) {
     g_pool_free.emplace(it->second, FreeBlock{p, s, e, false});
 }
 
+void pool_free_idle(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_size.find(p);
+    if (it == g_pool_size.end()) return;
+    g_pool_free.emplace(it->second, FreeBlock{p, nullptr, nullptr, false});
+}
+
 void pool_release_idle() {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     for (auto it = g_pool_free.begin(); it != g_pool_free.end();) {

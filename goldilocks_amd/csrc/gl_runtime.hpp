@@ -44,6 +44,8 @@ int ensure_init();
 void *pool_alloc(size_t bytes, hipStream_t s);
 // Return a block; work enqueued on s before this call may still use it.
 void pool_free(void *p, hipStream_t s);
+// Return a block no queued work uses any more: any stream may take it at once.
+void pool_free_idle(void *p);
 // Free every idle block whose last use has completed (no device sync).
 void pool_release_idle();
 // The next `count` pool_alloc calls return nullptr (error-path tests).
@@ -58,6 +60,17 @@ struct DevBuf {
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { if (p) pool_free(p, s); }
+    // Allocate into an empty buffer.
+    void alloc(size_t bytes, hipStream_t stream) {
+        s = stream;
+        p = bytes ? pool_alloc(bytes, stream) : nullptr;
+    }
+    // Give the block back as idle: the caller has seen every use of it
+    // complete (e.g. a synchronous copy ordered after them on its stream).
+    void release_idle() {
+        if (p) pool_free_idle(p);
+        p = nullptr;
+    }
     template <class T> T *as() const { return (T *)p; }
 };
 
