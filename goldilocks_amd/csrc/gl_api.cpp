@@ -87,6 +87,16 @@ int row_tree_dev_impl(const u64 *d_rows, size_t nrows, size_t cols, uint8_t *d_o
     return GL_OK;
 }
 
+// Copy a host entry point's result back (synchronously: it completes after
+// every earlier use of the call's blocks on s), then return those blocks to
+// the pool idle, with no event to record.
+template <class... Bufs>
+int copy_back_idle(void *h_dst, const void *d_src, size_t bytes, hipStream_t s, Bufs &...bufs) {
+    const int rc = copy_d2h(h_dst, d_src, bytes, s);
+    if (rc == GL_OK) (bufs.release_idle(), ...);
+    return rc;
+}
+
 // scratch: where the ping-pong buffer goes when the caller frees it (e.g. as
 // idle after a synchronous copy); by default the call's own, stream ordered.
 int ntt_dev_impl(u64 *d, u64 n, u64 S0, u64 count, bool inverse, hipStream_t s, DevBuf *scratch = nullptr) {
@@ -422,12 +432,7 @@ int gl_ntt_batch(uint64_t *v, size_t n, size_t count, int inverse) {
     if (!d.p) return set_error(GL_E_NOMEM, "device allocation failed");
     if ((rc = copy_h2d(d.p, v, bytes, s))) return rc;
     if ((rc = ntt_dev_impl(d.as<u64>(), n, 1, count, inverse != 0, s, &tmp))) return rc;
-    if ((rc = copy_d2h(v, d.p, bytes, s))) return rc;
-    // the synchronous copy back ran after every use of both blocks on s:
-    // they return to the pool idle (no event to record)
-    d.release_idle();
-    tmp.release_idle();
-    return GL_OK;
+    return copy_back_idle(v, d.p, bytes, s, d, tmp);
 }
 
 int gl_ntt(uint64_t *v, size_t n) { return gl_ntt_batch(v, n, 1, 0); }
@@ -601,7 +606,7 @@ int gl_k12_layer(const uint8_t *in, size_t len, uint8_t *out) {
     if ((rc = copy_h2d(din, in, len, s))) return rc;
     launch_k12_layer(din, len, dout, stage, false, nullptr, s);
     GL_HIP(hipGetLastError());
-    return copy_d2h(out, dout, olen, s);
+    return copy_back_idle(out, dout, olen, s, d);
 }
 
 // ---------------------------------------------------------------- openings
@@ -680,7 +685,7 @@ int gl_merkle_tree(const uint8_t *in, size_t len, uint8_t *out, size_t cap, size
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
     if ((rc = copy_h2d(din, in, len, s))) return rc;
     if ((rc = merkle_dev_impl(din, len, dout, need, nullptr, s))) return rc;
-    return copy_d2h(out, dout, need, s);
+    return copy_back_idle(out, dout, need, s, d);
 }
 
 int gl_pow_solve(const uint8_t *challenge32, uint64_t threshold, uint64_t max_nonce, uint64_t *nonce) {
@@ -793,11 +798,11 @@ int gl_transpose(uint64_t *m, size_t rows, size_t cols) {
     if (rows == cols) {
         launch_transpose_square(a.as<u64>(), rows, s);
         GL_HIP(hipGetLastError());
-        return copy_d2h(m, a.p, bytes, s);
+        return copy_back_idle(m, a.p, bytes, s, a);
     }
     launch_transpose(a.as<u64>(), b.as<u64>(), rows, cols, s);
     GL_HIP(hipGetLastError());
-    return copy_d2h(m, b.p, bytes, s);
+    return copy_back_idle(m, b.p, bytes, s, a, b);
 }
 
 static int bit_reverse_log2(size_t n, int *k) {
@@ -834,7 +839,7 @@ int gl_bit_reverse(uint64_t *v, size_t n) {
     if ((rc = copy_h2d(d.p, v, n * 8, s))) return rc;
     launch_bit_reverse(d.as<u64>(), n, k, s);
     GL_HIP(hipGetLastError());
-    return copy_d2h(v, d.p, n * 8, s);
+    return copy_back_idle(v, d.p, n * 8, s, d);
 }
 
 int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
@@ -849,7 +854,7 @@ int gl_convolve(uint64_t *a, const uint64_t *b_ntt, size_t n) {
     if ((rc = copy_h2d(da, a, n * 8, s))) return rc;
     if ((rc = copy_h2d(db, b_ntt, n * 8, s))) return rc;
     if ((rc = gl_convolve_dev(da, db, n, s))) return rc;
-    return copy_d2h(a, da, n * 8, s);
+    return copy_back_idle(a, da, n * 8, s, d);
 }
 
 }  // extern "C"
@@ -891,7 +896,7 @@ int commit_host(const uint64_t *m, size_t rows, size_t cols, unsigned blowup, ui
     if ((rc = commit_dev(dm.as<u64>(), rows, cols, blowup, dl.as<u64>(), dt.as<uint8_t>(), need, nullptr, s, tree)))
         return rc;
     if ((rc = copy_d2h(lde_out, dl.p, ext * cols * 8, s))) return rc;
-    return copy_d2h(layers_out, dt.p, need, s);
+    return copy_back_idle(layers_out, dt.p, need, s, dm, dl, dt);
 }
 
 }  // namespace
@@ -952,7 +957,7 @@ int gl_row_tree(const uint64_t *rows, size_t nrows, size_t cols, uint8_t *layers
     uint8_t *din = d.as<uint8_t>(), *dout = din + in_pad;
     if ((rc = copy_h2d(din, rows, in_bytes, s))) return rc;
     if ((rc = row_tree_dev_impl((const u64 *)din, nrows, cols, dout, need, nullptr, s))) return rc;
-    return copy_d2h(layers_out, dout, need, s);
+    return copy_back_idle(layers_out, dout, need, s, d);
 }
 
 size_t gl_row_tree_depth(size_t leaves) {
