@@ -161,7 +161,12 @@ int gl_server::start_job(Slot &slot, const std::shared_ptr<Job> &job) {
 void gl_server::finish_job(Slot &slot, int status) {
     std::shared_ptr<Job> job = slot.job;
     slot.job.reset();
-    for (void *p : job->bufs) pool_free(p, slot.stream);
+    // a finished job's event has completed: its blocks go back idle (no event
+    // to record); after a failure they stay ordered on the slot's stream
+    for (void *p : job->bufs) {
+        if (status == GL_OK) pool_free_idle(p);
+        else pool_free(p, slot.stream);
+    }
     job->bufs.clear();
     job->keep.clear();
     job->plan.reset();
