@@ -6,7 +6,9 @@ way a block is never handed to a second stream while work on the first may
 still use it.  The transforms below alternate between the NULL stream and
 two side streams with no synchronization in between, with equal lengths (so
 every call asks the pool for the same scratch size), and every result must
-equal the oracle's.
+equal the oracle's.  Host-slice calls return their blocks idle after their
+synchronous copy back; a thread making them runs beside a thread that keeps
+a side stream busy with the same length.
 """
 import pytest
 
@@ -80,3 +82,57 @@ def test_null_stream_blocks_after_side_stream_use(gl, oracle):
     torch.cuda.synchronize()
     for name, t in (("side", a), ("null", b), ("side again", c)):
         assert (from_torch(t) == expect).all(), name
+
+
+def test_host_slice_calls_beside_side_stream_work(gl, oracle):
+    """Host-slice transforms (their blocks go back to the pool idle after
+    the synchronous copy back) in one thread while another thread keeps a
+    side stream busy with device transforms of the same length (the same
+    scratch size class): every result equals the oracle's."""
+    import threading
+
+    import numpy as np
+    import torch
+    n = 1 << 16
+    note("pool: host-slice calls beside side-stream work, 2^16")
+    rng = np.random.default_rng(55)
+    hx = (rng.integers(0, 2**63, n, dtype=np.uint64) * 2 + 1) % np.uint64(0xFFFFFFFF00000001)
+    hexp = oracle.ntt(hx)
+    dx = torch.empty(n, dtype=torch.int64, device="cuda")
+    gl.fill_random(dx, 66)
+    torch.cuda.synchronize()
+    dexp = oracle.ntt(from_torch(dx))
+    errors = []
+
+    def host_loop():
+        try:
+            for i in range(60):
+                v = hx.copy()
+                gl.ntt(v)
+                if not np.array_equal(v, hexp):
+                    errors.append(f"host call {i} differs")
+                    return
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    def device_loop():
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for i in range(60):
+                    y = dx.clone()
+                    gl.ntt(y)
+                    s.synchronize()
+                    if not np.array_equal(from_torch(y), dexp):
+                        errors.append(f"device call {i} differs")
+                        return
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=host_loop), threading.Thread(target=device_loop)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=300)
+    assert not any(t.is_alive() for t in ts), "a thread did not finish"
+    assert not errors, errors[0]
